@@ -1,0 +1,177 @@
+#!/usr/bin/env python
+"""Benchmark: incoherent closest-hit rays on the displaced level-7 icosphere.
+
+Workload (one step): 2^24 rays, origins uniform in the mesh's world bounds,
+directions uniform on the sphere, traced for the closest hit against the
+icosphere (327,680 tris) inside the Cornell enclosure (36 more tris). Rays
+and the scene are resident in HBM before the timed region. value = rays /
+second over K timed launches (HIP events on the launch stream).
+
+Roofline (HBM-bound path, no dense contraction): algorithmic bytes per ray
+  B_ray = 32 (ray in) + 16 (hit out) + 32 * N_node + 36 * N_tri
+where N_node / N_tri are the reference BVH's bounds / object tests per ray
+(muon/acceleration.cc:92,108), counted by the CPU restatement on a sample of
+the same rays; achieved = B_ray * rays / mean launch time, peak 8 TB/s.
+
+CPU baseline: the CPU restatement of the reference (same BVH, same
+per-primitive object-space tests) on a bounded sample of the same rays,
+timed on this host.
+
+Also reported (outside the timed region): one full path-traced render of the
+same scene (512x512, 64 spp, depth 8, BRDF IS + NEE + MIS).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s (incoherent, 1 GPU) + p50/p99 render latency at fixed request rate"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-render", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from muon_amd import Context, DeviceScene, HostScene, scenes
+    from muon_amd import _lib
+    from muon_amd.renderer import render_device
+
+    dev = torch.device("cuda:0")
+    log("bench: building the level-7 icosphere scene")
+    text = scenes.icosphere_text()
+    hs = HostScene.from_text(text)
+    ctx = Context(0)
+    ds = DeviceScene(ctx, hs)
+    info = ds.info()
+    lo, hi = scenes.icosphere_world_bounds()
+    o_np, d_np = scenes.microbench_rays(args.rays, lo, hi, seed=0)
+    o = torch.from_numpy(o_np).to(dev)
+    d = torch.from_numpy(d_np).to(dev)
+    prim = torch.empty(args.rays, dtype=torch.int32, device=dev)
+    t = torch.empty(args.rays, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    log(f"bench: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        ds.intersect_closest(o, d, None, prim, t)
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.steps} launches of {args.rays} rays")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        ds.intersect_closest(o, d, None, prim, t)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    mean_ms = float(np.mean(launch_ms))
+    total_s = sum(launch_ms) / 1e3
+    mrays = args.rays * args.steps / total_s / 1e6
+    hits = int((prim >= 0).sum().item())
+
+    # device traversal counters for one launch
+    import ctypes as C
+    ctr = torch.zeros(2, dtype=torch.int64, device=dev)
+    _lib.check(_lib.lib.muonx_intersect_closest_counted(ds.h, o.data_ptr(), d.data_ptr(), None, args.rays,
+                                                        prim.data_ptr(), t.data_ptr(), ctr.data_ptr(),
+                                                        stream.cuda_stream))
+    torch.cuda.synchronize()
+    dev_nodes, dev_prims = [int(x) for x in ctr.cpu().tolist()]
+
+    # reference-BVH counters and CPU baseline from the CPU restatement
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    orc = _oracle.Oracle()
+    oscene = orc.load_text(text)
+    sample = 1 << 16
+    _, _, _, _, st = oscene.intersect(o_np[:sample], d_np[:sample], threads=min(16, os.cpu_count() or 8))
+    n_node = st[4] / sample
+    n_obj = st[2] / sample
+    b_ray = 32 + 16 + 32 * n_node + 36 * n_obj
+    achieved = b_ray * args.rays / (mean_ms / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_closest_icosphere.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_ray": round(b_ray, 1), "ref_bounds_tests_per_ray": round(n_node, 2),
+                "ref_object_tests_per_ray": round(n_obj, 2),
+                "device_nodes_per_ray": round(dev_nodes / args.rays, 2),
+                "device_prim_tests_per_ray": round(dev_prims / args.rays, 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 8)
+        done, secs, chunk = 0, 0.0, 1 << 18
+        while secs < args.cpu_seconds and done < args.rays:
+            n = min(chunk, args.rays - done)
+            t0 = time.perf_counter()
+            oscene.intersect(o_np[done:done + n], d_np[done:done + n], threads=threads)
+            secs += time.perf_counter() - t0
+            done += n
+        cpu = {"value": round(done / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"first {done} of the same microbench rays, closest hit, {threads} threads, {secs:.1f} s"}
+        log(f"bench: cpu baseline {cpu['value']} Mrays/s")
+
+    render = None
+    if not args.no_render:
+        log("bench: path-traced render 512x512 64spp depth 8")
+        accum, rst = render_device(ds, hs.params)  # warm the workspace
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        accum, rst = render_device(ds, hs.params)
+        torch.cuda.synchronize()
+        rs = time.perf_counter() - t0
+        rays_total = rst.primary_rays + rst.secondary_rays + rst.shadow_rays
+        render = {"seconds": round(rs, 4), "images_per_s": round(1.0 / rs, 3),
+                  "mrays_per_s": round(rays_total / rs / 1e6, 1), "primary_rays": rst.primary_rays,
+                  "secondary_rays": rst.secondary_rays, "shadow_rays": rst.shadow_rays,
+                  "mean_radiance": float(accum.mean().item()) / hs.params.spp}
+
+    out = {
+        "metric": METRIC, "value": round(mrays, 1), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(mean_ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "closest-hit microbench: 2^24 incoherent rays vs displaced level-7 icosphere "
+                               "(327,680 tris) in the Cornell enclosure",
+                   "rays_per_step": args.rays, "prims": info["num_prims"], "bvh_depth": info["bvh_depth"],
+                   "hit_fraction": round(hits / args.rays, 4), "wall_s": round(wall, 3)},
+        "roofline": roofline, "cpu_baseline": cpu, "render": render,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

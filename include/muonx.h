@@ -1,0 +1,219 @@
+/* muonx — MI355X (gfx950) wavefront path tracer behind muon's render surface.
+ *
+ * C ABI: plain C types, opaque handles, int status codes (0 = ok, < 0 =
+ * error; muonx_last_error() returns a thread-local message). No exception
+ * crosses this boundary. The caller owns host arrays; the library owns the
+ * device memory of a scene until muonx_scene_destroy().
+ *
+ * The reference (voithos/muon) has no C ABI; its seams are C++ interfaces.
+ * Each entry point below names the reference interface it replaces.
+ */
+#ifndef MUONX_H_
+#define MUONX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MUONX_OK 0
+#define MUONX_ERR_INVALID (-1)   /* bad argument */
+#define MUONX_ERR_HIP (-2)       /* HIP runtime error (no device, launch failure) */
+#define MUONX_ERR_NOMEM (-3)     /* device or host allocation failed */
+#define MUONX_ERR_IO (-4)        /* file could not be read */
+#define MUONX_ERR_STATE (-5)     /* call not valid in the object's state */
+#define MUONX_ERR_CAPACITY (-6)  /* request exceeds an admission limit */
+
+/* Integrator selection (muon/parser.cc:330-355). */
+#define MUONX_INTEGRATOR_NORMALS 0
+#define MUONX_INTEGRATOR_ALBEDO 1
+#define MUONX_INTEGRATOR_DEPTH 2
+#define MUONX_INTEGRATOR_RAYTRACER 3
+#define MUONX_INTEGRATOR_ANALYTICDIRECT 4
+#define MUONX_INTEGRATOR_PATHTRACER 5
+
+/* muon/nee.h, muon/importance_sampling.h */
+#define MUONX_NEE_OFF 0
+#define MUONX_NEE_ON 1
+#define MUONX_NEE_MIS 2
+#define MUONX_IS_HEMISPHERE 0
+#define MUONX_IS_COSINE 1
+#define MUONX_IS_BRDF 2
+
+/* Material BRDF tags. "lambertian" in a scene file yields PHONG
+ * (muon/parser.cc:165-178 falls through). */
+#define MUONX_BRDF_LAMBERTIAN 0
+#define MUONX_BRDF_PHONG 1
+#define MUONX_BRDF_GGX 2
+
+#define MUONX_LIGHT_DIRECTIONAL 0
+#define MUONX_LIGHT_POINT 1
+#define MUONX_LIGHT_QUAD 2
+
+#define MUONX_PRIM_TRI 0
+#define MUONX_PRIM_SPHERE 1
+
+typedef struct muonx_ctx muonx_ctx;
+typedef struct muonx_scene muonx_scene;
+typedef struct muonx_host_scene muonx_host_scene;
+typedef struct muonx_server muonx_server;
+
+/* Renderer options; replaces muon::Options (muon/options.h:11-23). */
+typedef struct muonx_options {
+  int32_t acceleration;       /* 0 = linear, 1 = bvh (muon/acceleration_type.h:11-17) */
+  int32_t partition_strategy; /* 0 = uniform, 1 = midpoint, 2 = sah */
+  int32_t parallelism;        /* host threads for CPU-side work */
+  int32_t show_stats;
+  int64_t default_seed;       /* >= 0 replaces the std::random_device seed (muon/parser.cc:181) */
+} muonx_options;
+
+/* Scene descriptor handed to the device: host arrays in the reference's data
+ * model (muon/scene.h:22-95, muon/objects.h, muon/materials.h:125-148,
+ * muon/lighting.h), plus the binary SAH BVH built on the host
+ * (muon/acceleration.cc:204-458) in pre-order. Primitive ids are parse
+ * (AddPrimitive) order. All arrays are read during muonx_scene_create only. */
+typedef struct muonx_scene_desc {
+  uint32_t num_prims;
+  const uint8_t *prim_type;        /* MUONX_PRIM_* */
+  const float *prim_geom;          /* 12 per prim: tri v0,v1,v2 (object space); sphere cx,cy,cz,r */
+  const float *prim_vnormal;       /* 9 per prim: vertex normals N0,N1,N2, or NULL */
+  const uint8_t *prim_use_vnormal; /* 1 = interpolate vertex normals (muon/objects.cc:163-166), or NULL */
+  const int32_t *prim_transform;   /* index into transforms */
+  const int32_t *prim_material;    /* index into materials */
+  const int32_t *prim_light;       /* quad-light index for the emitter tris, else -1 */
+
+  uint32_t num_transforms;
+  const float *transforms;         /* 48 per transform: M, M^-1, M^-T, column-major */
+
+  uint32_t num_materials;
+  const float *materials;          /* 16 per material: ambient3 diffuse3 specular3 emission3
+                                      shininess roughness brdf(MUONX_BRDF_*) 0 */
+  uint32_t num_lights;
+  const float *lights;             /* 16 per light: kind color3 a3 b3 c3 0 — directional: a=direction;
+                                      point: a=position b=attenuation; quad: a=corner b=edge0 c=edge1 */
+  uint32_t num_nodes;              /* binary BVH, pre-order, root = 0 */
+  const float *node_bounds;        /* 6 per node: min xyz, max xyz */
+  const int32_t *node_left;        /* child index, -1 for a leaf */
+  const int32_t *node_right;
+  const int32_t *node_axis;        /* split axis of an internal node */
+  const int32_t *node_start;       /* leaf: first index into leaf_prims */
+  const int32_t *node_count;       /* leaf: primitive count; 0 for internal nodes */
+  const int32_t *leaf_prims;       /* num_prims parse-order ids in leaf order */
+} muonx_scene_desc;
+
+/* Per-render settings: the scene's render properties (muon/scene.h:31-51)
+ * and the pinhole camera precomputed on the host (muon/camera.cc:15-35). */
+typedef struct muonx_render_params {
+  int32_t width, height, spp;
+  int32_t min_depth, max_depth;
+  int32_t integrator;           /* MUONX_INTEGRATOR_* */
+  int32_t nee;                  /* MUONX_NEE_* */
+  int32_t importance_sampling;  /* MUONX_IS_* */
+  int32_t russian_roulette;
+  int32_t light_samples;
+  int32_t light_stratify;
+  uint32_t seed;                /* path-sampling seed (muon/parser.cc:347-349) */
+  float gamma;
+  float eye[3], cam_u[3], cam_v[3], cam_w[3];
+  float tan_fov_x, tan_fov_y;
+  int32_t has_camera;
+  /* 0: sample 0 of a pixel at its centre, later samples jittered by the
+     counter-based RNG (statistically the reference's Sampler,
+     muon/sampling.cc:86-108). */
+  int32_t sample_mode;
+} muonx_render_params;
+
+/* Counters of muon's TraceStats (muon/stats.h:11-44) plus shadow rays, which
+ * the reference does not count. */
+typedef struct muonx_trace_stats {
+  uint64_t primary_rays;
+  uint64_t secondary_rays;
+  uint64_t shadow_rays;
+  uint64_t render_ns;  /* device time of the render, ns */
+} muonx_trace_stats;
+
+typedef struct muonx_scene_info {
+  uint64_t device_bytes;  /* HBM footprint of the resident scene */
+  uint32_t num_prims;
+  uint32_t num_nodes;     /* device BVH nodes */
+  uint32_t bvh_depth;     /* binary SAH tree depth */
+  uint32_t num_lights;
+  uint32_t num_quad_lights;
+} muonx_scene_info;
+
+/* ------------------------------------------------------------ misc */
+const char *muonx_last_error(void);
+const char *muonx_version(void);
+
+/* ------------------------------------------------------------ host ingest (CPU)
+ * Replaces muon::Parser::Parse (muon/parser.cc:230-880) together with
+ * acceleration::BVH::Init (muon/acceleration.cc:204-229): parses a .muon
+ * scene and builds the SAH BVH on the host. Needs no GPU. */
+int muonx_host_scene_parse_file(const char *path, const muonx_options *opts, muonx_host_scene **out);
+int muonx_host_scene_parse_text(const char *text, size_t len, const muonx_options *opts,
+                                muonx_host_scene **out);
+void muonx_host_scene_destroy(muonx_host_scene *hs);
+/* Views into the host scene; valid until muonx_host_scene_destroy(). */
+int muonx_host_scene_desc(const muonx_host_scene *hs, muonx_scene_desc *out);
+int muonx_host_scene_params(const muonx_host_scene *hs, muonx_render_params *out);
+/* Count of malformed / unknown lines (muon logs them and continues). */
+int muonx_host_scene_warnings(const muonx_host_scene *hs);
+
+/* ------------------------------------------------------------ device context / scene */
+/* One context per device; calls on a context are serialised internally. */
+int muonx_ctx_create(int device, muonx_ctx **out);
+void muonx_ctx_destroy(muonx_ctx *ctx);
+
+/* Uploads a scene; replaces the Structure::Init()'d acceleration structure
+ * (muon/acceleration.h:26-53) with a device-resident BVH. Immutable after
+ * creation and shareable by concurrent renders. */
+int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *desc, muonx_scene **out);
+void muonx_scene_destroy(muonx_scene *scene);
+int muonx_scene_get_info(const muonx_scene *scene, muonx_scene_info *out);
+
+/* ------------------------------------------------------------ queries (device pointers)
+ * Batched acceleration::Structure::Intersect (muon/acceleration.cc:65-142).
+ * ray_o / ray_d: 3n floats each (xyz per ray); tmax: n floats or NULL (+inf).
+ * prim_id: parse-order id or -1 on a miss; t: world distance. */
+int muonx_intersect_closest(muonx_scene *scene, const float *ray_o, const float *ray_d, const float *tmax,
+                            uint32_t n, int32_t *prim_id, float *t, void *stream);
+/* Batched acceleration::Structure::HasIntersection (muon/acceleration.cc:144-202). */
+int muonx_intersect_any(muonx_scene *scene, const float *ray_o, const float *ray_d, const float *tmax,
+                        uint32_t n, uint8_t *occluded, void *stream);
+
+/* muonx_intersect_closest plus traversal counters, the device analogue of
+ * the TraceStats increments in BVH::Intersect (muon/acceleration.cc:92,101,108,114):
+ * counters[0] += internal nodes fetched (two child boxes each),
+ * counters[1] += primitive tests. counters: device pointer to 2 uint64. */
+int muonx_intersect_closest_counted(muonx_scene *scene, const float *ray_o, const float *ray_d, const float *tmax,
+                                    uint32_t n, int32_t *prim_id, float *t, unsigned long long *counters,
+                                    void *stream);
+
+/* ------------------------------------------------------------ render
+ * Replaces the tile loop of muon::Renderer::Render (muon/renderer.cc:40-76):
+ * Camera::CastRay + Integrator::Trace per sample + Film::SetPixel.
+ * accum_rgb: device pointer to H*W*3 floats, row-major, receives the
+ * unnormalised sum over samples (zeroed by the call). stats may be NULL. */
+int muonx_render(muonx_scene *scene, const muonx_render_params *params, float *accum_rgb,
+                 muonx_trace_stats *stats, void *stream);
+
+/* muonx_render into a host buffer: the library stages the film in its own
+ * device memory and copies it back (for callers without a HIP toolchain,
+ * such as the muonx CLI). accum_rgb: host pointer, H*W*3 floats. */
+int muonx_render_host(muonx_scene *scene, const muonx_render_params *params, float *accum_rgb,
+                      muonx_trace_stats *stats);
+
+/* Film output, muon::Film::WriteOutput (muon/film.cc:21-43): /spp,
+ * pow(v, 1/gamma), clamp [0,1], *255 truncated. Host arrays. */
+int muonx_film_to_rgb8(const float *accum_rgb, int32_t width, int32_t height, int32_t spp, float gamma,
+                       uint8_t *rgb8);
+/* Writes an 8-bit RGB PNG (the reference saves through CImg). */
+int muonx_write_png(const char *path, const uint8_t *rgb8, int32_t width, int32_t height);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUONX_H_ */

@@ -1,0 +1,101 @@
+"""ctypes bindings of include/muonx.h (libmuonx.so)."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(HERE, "lib", "libmuonx.so")
+
+
+class MuonxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"muonx error {code}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [("acceleration", C.c_int32), ("partition_strategy", C.c_int32), ("parallelism", C.c_int32),
+                ("show_stats", C.c_int32), ("default_seed", C.c_int64)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("num_prims", C.c_uint32), ("prim_type", C.c_void_p), ("prim_geom", C.c_void_p),
+        ("prim_vnormal", C.c_void_p), ("prim_use_vnormal", C.c_void_p), ("prim_transform", C.c_void_p),
+        ("prim_material", C.c_void_p), ("prim_light", C.c_void_p),
+        ("num_transforms", C.c_uint32), ("transforms", C.c_void_p),
+        ("num_materials", C.c_uint32), ("materials", C.c_void_p),
+        ("num_lights", C.c_uint32), ("lights", C.c_void_p),
+        ("num_nodes", C.c_uint32), ("node_bounds", C.c_void_p), ("node_left", C.c_void_p),
+        ("node_right", C.c_void_p), ("node_axis", C.c_void_p), ("node_start", C.c_void_p),
+        ("node_count", C.c_void_p), ("leaf_prims", C.c_void_p),
+    ]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("min_depth", C.c_int32),
+        ("max_depth", C.c_int32), ("integrator", C.c_int32), ("nee", C.c_int32),
+        ("importance_sampling", C.c_int32), ("russian_roulette", C.c_int32), ("light_samples", C.c_int32),
+        ("light_stratify", C.c_int32), ("seed", C.c_uint32), ("gamma", C.c_float),
+        ("eye", C.c_float * 3), ("cam_u", C.c_float * 3), ("cam_v", C.c_float * 3), ("cam_w", C.c_float * 3),
+        ("tan_fov_x", C.c_float), ("tan_fov_y", C.c_float), ("has_camera", C.c_int32),
+        ("sample_mode", C.c_int32),
+    ]
+
+
+class TraceStatsC(C.Structure):
+    _fields_ = [("primary_rays", C.c_uint64), ("secondary_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("render_ns", C.c_uint64)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("device_bytes", C.c_uint64), ("num_prims", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("bvh_depth", C.c_uint32), ("num_lights", C.c_uint32), ("num_quad_lights", C.c_uint32)]
+
+
+P = C.c_void_p
+# (name, restype, argtypes): every entry point declared in include/muonx.h
+SIGNATURES = [
+    ("muonx_last_error", C.c_char_p, []),
+    ("muonx_version", C.c_char_p, []),
+    ("muonx_host_scene_parse_file", C.c_int, [C.c_char_p, C.POINTER(Options), C.POINTER(P)]),
+    ("muonx_host_scene_parse_text", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(Options), C.POINTER(P)]),
+    ("muonx_host_scene_destroy", None, [P]),
+    ("muonx_host_scene_desc", C.c_int, [P, C.POINTER(SceneDesc)]),
+    ("muonx_host_scene_params", C.c_int, [P, C.POINTER(RenderParams)]),
+    ("muonx_host_scene_warnings", C.c_int, [P]),
+    ("muonx_ctx_create", C.c_int, [C.c_int, C.POINTER(P)]),
+    ("muonx_ctx_destroy", None, [P]),
+    ("muonx_scene_create", C.c_int, [P, C.POINTER(SceneDesc), C.POINTER(P)]),
+    ("muonx_scene_destroy", None, [P]),
+    ("muonx_scene_get_info", C.c_int, [P, C.POINTER(SceneInfo)]),
+    ("muonx_intersect_closest", C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
+    ("muonx_intersect_any", C.c_int, [P, P, P, P, C.c_uint32, P, P]),
+    ("muonx_intersect_closest_counted", C.c_int, [P, P, P, P, C.c_uint32, P, P, P, P]),
+    ("muonx_render", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC), P]),
+    ("muonx_render_host", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC)]),
+    ("muonx_film_to_rgb8", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),
+    ("muonx_write_png", C.c_int, [C.c_char_p, P, C.c_int32, C.c_int32]),
+]
+
+
+def _load():
+    if not os.path.exists(library_path):
+        raise ImportError(
+            f"{library_path} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(library_path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc != 0:
+        raise MuonxError(rc, lib.muonx_last_error().decode(errors="replace"))
+    return rc

@@ -1,0 +1,58 @@
+// Device-resident scene layout (HBM, struct-of-arrays).
+//
+// BVH: the host's binary SAH tree with each node's two child boxes stored in
+// the parent ("child boxes in parent"), 64 B per internal node = 4 x float4:
+//   n0 = (c0.min.x, c0.max.x, c0.min.y, c0.max.y)
+//   n1 = (c1.min.x, c1.max.x, c1.min.y, c1.max.y)
+//   n2 = (c0.min.z, c0.max.z, c1.min.z, c1.max.z)
+//   n3 = (ref0, ref1, meta0, meta1) as ints
+// ref_i is the child's node index, or for a leaf the first primitive (leaf
+// order). meta_i bit 31 marks a leaf, bits 0..28 its primitive count;
+// bits 29..30 of meta0 hold this node's split axis. One node fetch tests both
+// children, and the near child (reference order: children[dir[axis] < 0]
+// first, muon/acceleration.cc:77-82,134-138) is chosen from the axis.
+//
+// Primitives (leaf order), 48 B = 3 x float4:
+//   tri:    (v0.xyz, tid)  (v1.xyz, 0)  (v2.xyz, 0)     object space
+//   sphere: (c.xyz, tid | kSphereBit)  (r, 0, 0, 0)  (0)
+// tid indexes 48-float transform records (M, M^-1, M^-T column-major).
+#ifndef MUONX_DEVICE_DEV_SCENE_H_
+#define MUONX_DEVICE_DEV_SCENE_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace muonx {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kCountMask = 0x1fffffffu;
+constexpr uint32_t kSphereBit = 0x80000000u;
+
+// Material record, 16 floats: ambient3 diffuse3 specular3 emission3
+// shininess roughness brdf reflectiveness (precomputed, muon/materials.cc:113-123,270-293).
+constexpr int kMatStride = 16;
+// Light record, 16 floats: kind color3 a3 b3 c3 area(quad) — see muonx_scene_desc.
+constexpr int kLightStride = 16;
+
+struct DevSceneView {
+  const float4 *nodes;      // 4 per internal node
+  const float4 *prims;      // 3 per primitive, leaf order
+  const float *xforms;      // 48 per transform
+  const int32_t *prim_id;   // leaf order -> parse order
+  const int32_t *prim_mat;  // leaf order -> material
+  const int32_t *prim_light;
+  const float *prim_vn;     // 9 per primitive (leaf order) or nullptr
+  const uint8_t *prim_use_vn;
+  const float *materials;
+  const float *lights;
+  uint32_t num_prims;
+  uint32_t num_nodes;
+  uint32_t num_lights;
+  uint32_t num_quad_lights;
+  uint32_t bvh_depth;
+};
+
+}  // namespace muonx
+
+#endif

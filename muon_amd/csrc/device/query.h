@@ -1,0 +1,17 @@
+#ifndef MUONX_DEVICE_QUERY_H_
+#define MUONX_DEVICE_QUERY_H_
+
+#include <hip/hip_runtime.h>
+
+#include "dev_scene.h"
+
+namespace muonx {
+
+constexpr int kQueryBlock = 128;
+
+int LaunchQuery(const DevSceneView &s, bool any, const float *ro, const float *rd, const float *tmax, uint32_t n,
+                int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters, hipStream_t stream);
+
+}  // namespace muonx
+
+#endif

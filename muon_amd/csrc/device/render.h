@@ -1,0 +1,74 @@
+// Shared pieces of the GPU render path: camera, sample positions, the
+// per-context workspace of queues and counters.
+#ifndef MUONX_DEVICE_RENDER_H_
+#define MUONX_DEVICE_RENDER_H_
+
+#include "../common/vecmath.h"
+#include "rng.h"
+#include "runtime.h"
+
+namespace muonx {
+
+constexpr int kSimpleBlock = 128;
+
+// Device counters (u64): rays by kind, plus scratch.
+constexpr int kCtrPrimary = 0;
+constexpr int kCtrSecondary = 1;
+constexpr int kCtrShadow = 2;
+constexpr int kNumCounters = 8;
+
+struct CamParams {
+  V3 eye, u, v, w;
+  float tan_fov_x, tan_fov_y;
+  int width, height;
+};
+
+// muon/camera.cc:37-62
+__device__ __forceinline__ void cast_ray(const CamParams &c, float x, float y, V3 &o, V3 &d) {
+  const float hw = c.width / 2.0f;
+  const float hh = c.height / 2.0f;
+  const float ox = (x - hw) / hw;
+  const float oy = (hh - y) / hh;
+  const float alpha = c.tan_fov_x * ox;
+  const float beta = c.tan_fov_y * oy;
+  d = normalize((alpha * c.u + beta * c.v) - c.w);
+  o = c.eye;
+}
+
+// muon/sampling.cc:86-108: sample 0 at the pixel centre, later samples
+// jittered uniformly inside the pixel (counter-based draws instead of the
+// per-tile mt19937 stream), with the same guard against rounding onto the
+// next pixel.
+__device__ __forceinline__ void sample_position(int px, int py, int sidx, uint32_t key, float &x, float &y) {
+  if (sidx == 0) {
+    x = float(px) + 0.5f;
+    y = float(py) + 0.5f;
+    return;
+  }
+  uint32_t ctr = 0x40000000u;  // draw indices reserved for the sampler
+  const float u = rng_uniform(key, ctr);
+  const float v = rng_uniform(key, ctr);
+  x = float(px) + u;
+  y = float(py) + v;
+  if (x == float(px) + 1.0f) x = nextafterf(x, -__builtin_inff());
+  if (y == float(py) + 1.0f) y = nextafterf(y, -__builtin_inff());
+}
+
+// Per-context device workspace: grown on demand, reused across renders.
+struct RenderWorkspace {
+  DevBuf counters;
+  DevBuf paths;     // path-state SoA
+  DevBuf queues;    // ray queues
+  DevBuf misc;
+  DevBuf film;      // staging film of muonx_render_host
+  size_t path_capacity = 0;
+};
+
+RenderWorkspace *GetWorkspace(muonx_ctx *ctx);
+CamParams MakeCam(const muonx_render_params &p);
+int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *accum, RenderWorkspace *ws,
+                     hipStream_t stream);
+
+}  // namespace muonx
+
+#endif

@@ -1,0 +1,79 @@
+// Library-internal runtime objects behind the opaque C handles.
+#ifndef MUONX_DEVICE_RUNTIME_H_
+#define MUONX_DEVICE_RUNTIME_H_
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/muonx.h"
+#include "dev_scene.h"
+
+namespace muonx {
+
+void SetError(const std::string &msg);
+
+#define MX_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      ::muonx::SetError(std::string(#call) + ": " + hipGetErrorString(e_));                \
+      return MUONX_ERR_HIP;                                                               \
+    }                                                                                     \
+  } while (0)
+
+// Device allocation that frees itself.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { Free(); }
+  void Free() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  hipError_t Alloc(size_t n) {
+    Free();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  // Grows (without preserving contents) to at least n bytes.
+  hipError_t Reserve(size_t n) {
+    if (n <= bytes && ptr) return hipSuccess;
+    return Alloc(n);
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+};
+
+struct RenderWorkspace;
+
+}  // namespace muonx
+
+struct muonx_ctx {
+  int device = 0;
+  std::mutex mu;
+  muonx::RenderWorkspace *ws = nullptr;
+  int num_cus = 0;
+};
+
+struct muonx_scene {
+  muonx_ctx *ctx = nullptr;
+  muonx::DevBuf nodes, prims, xforms, prim_id, prim_mat, prim_light, prim_vn, prim_use_vn, materials, lights;
+  muonx::DevSceneView view{};
+  uint32_t bvh_depth = 0;
+  uint64_t device_bytes = 0;
+  std::vector<float> host_lights;     // 16 per light
+  std::vector<float> host_materials;  // 16 per material
+};
+
+#endif

@@ -1,0 +1,251 @@
+// Scene upload: converts the host descriptor (parse-order primitives + the
+// binary SAH BVH in pre-order) into the device layout of dev_scene.h.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../common/vecmath.h"
+#include "runtime.h"
+
+namespace muonx {
+namespace {
+
+float BitsAsFloat(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, sizeof(f));
+  return f;
+}
+
+// Reflectiveness of Phong (muon/materials.cc:113-123) and GGX (:270-293).
+float Reflectiveness(const float *m) {
+  const float s = ((0.0f + m[6]) + m[7] + m[8]) / 3.0f;
+  const float d = ((0.0f + m[3]) + m[4] + m[5]) / 3.0f;
+  const float denom = d + s;
+  if (int(m[14]) == MUONX_BRDF_GGX) {
+    if (denom == 0.0f) return 1.0f;
+    return gmax(s / denom, 0.25f);
+  }
+  return denom > 0.0f ? s / denom : 0.0f;
+}
+
+template <typename T>
+int Upload(DevBuf &buf, const std::vector<T> &host, uint64_t &total) {
+  MX_HIP(buf.Alloc(host.size() * sizeof(T)));
+  if (!host.empty()) MX_HIP(hipMemcpy(buf.ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+  total += buf.bytes;
+  return MUONX_OK;
+}
+
+}  // namespace
+}  // namespace muonx
+
+using namespace muonx;
+
+extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muonx_scene **out) {
+  if (!ctx || !d || !out) {
+    SetError("muonx_scene_create: null argument");
+    return MUONX_ERR_INVALID;
+  }
+  *out = nullptr;
+  const uint32_t np = d->num_prims;
+  if (np == 0 || d->num_nodes == 0) {
+    SetError("muonx_scene_create: scene has no primitives");  // the reference dereferences a null BVH root
+    return MUONX_ERR_INVALID;
+  }
+  if (!d->prim_type || !d->prim_geom || !d->prim_transform || !d->prim_material || !d->node_bounds ||
+      !d->node_left || !d->node_right || !d->node_axis || !d->node_start || !d->node_count || !d->leaf_prims ||
+      !d->transforms || !d->materials) {
+    SetError("muonx_scene_create: descriptor is missing arrays");
+    return MUONX_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  MX_HIP(hipSetDevice(ctx->device));
+
+  // ---- primitives in leaf order (+ one sentinel that never hits)
+  std::vector<float4> prims(size_t(np + 1) * 3, make_float4(0, 0, 0, 0));
+  std::vector<int32_t> pid(np + 1, -1), pmat(np + 1, 0), plight(np + 1, -1);
+  std::vector<float> pvn;
+  std::vector<uint8_t> puvn;
+  bool any_vn = false;
+  if (d->prim_use_vnormal && d->prim_vnormal)
+    for (uint32_t i = 0; i < np; ++i) any_vn |= d->prim_use_vnormal[i] != 0;
+  if (any_vn) {
+    pvn.assign(size_t(np + 1) * 9, 0.0f);
+    puvn.assign(np + 1, 0);
+  }
+  for (uint32_t k = 0; k < np; ++k) {
+    const int32_t id = d->leaf_prims[k];
+    if (id < 0 || uint32_t(id) >= np) {
+      SetError("muonx_scene_create: leaf_prims holds an invalid id");
+      return MUONX_ERR_INVALID;
+    }
+    const float *g = d->prim_geom + size_t(id) * 12;
+    const int32_t tid = d->prim_transform[id];
+    if (tid < 0 || uint32_t(tid) >= d->num_transforms) {
+      SetError("muonx_scene_create: invalid transform index");
+      return MUONX_ERR_INVALID;
+    }
+    const int32_t mid = d->prim_material[id];
+    if (mid < 0 || uint32_t(mid) >= d->num_materials) {
+      SetError("muonx_scene_create: invalid material index");
+      return MUONX_ERR_INVALID;
+    }
+    uint32_t tw = uint32_t(tid);
+    float4 *q = &prims[size_t(k) * 3];
+    if (d->prim_type[id] == MUONX_PRIM_SPHERE) {
+      tw |= kSphereBit;
+      q[0] = make_float4(g[0], g[1], g[2], BitsAsFloat(tw));
+      q[1] = make_float4(g[3], 0, 0, 0);
+    } else {
+      q[0] = make_float4(g[0], g[1], g[2], BitsAsFloat(tw));
+      q[1] = make_float4(g[3], g[4], g[5], 0);
+      q[2] = make_float4(g[6], g[7], g[8], 0);
+    }
+    pid[k] = id;
+    pmat[k] = mid;
+    plight[k] = d->prim_light ? d->prim_light[id] : -1;
+    if (any_vn) {
+      puvn[k] = d->prim_use_vnormal[id];
+      std::memcpy(&pvn[size_t(k) * 9], d->prim_vnormal + size_t(id) * 9, 9 * sizeof(float));
+    }
+  }
+  // Sentinel: transform 0, all-zero triangle (zero normal: never hit), end of leaf.
+  prims[size_t(np) * 3 + 1].w = BitsAsFloat(1u);
+
+  // ---- BVH: child boxes in parent; internal binary nodes in pre-order.
+  const uint32_t nn = d->num_nodes;
+  std::vector<int32_t> dev_index(nn, -1);
+  uint32_t n_internal = 0;
+  uint32_t depth = 1;
+  {
+    // depth by a pre-order walk
+    std::vector<uint32_t> dep(nn, 1);
+    for (uint32_t i = 0; i < nn; ++i) {
+      if (d->node_count[i] == 0) {
+        const int32_t l = d->node_left[i], r = d->node_right[i];
+        if (l <= int32_t(i) || r <= int32_t(i) || uint32_t(l) >= nn || uint32_t(r) >= nn) {
+          SetError("muonx_scene_create: BVH is not in pre-order");
+          return MUONX_ERR_INVALID;
+        }
+        dev_index[i] = int32_t(n_internal++);
+        dep[l] = dep[r] = dep[i] + 1;
+      } else {
+        const int64_t s = d->node_start[i], c = d->node_count[i];
+        if (s < 0 || c <= 0 || s + c > int64_t(np)) {
+          SetError("muonx_scene_create: leaf range out of bounds");
+          return MUONX_ERR_INVALID;
+        }
+        prims[size_t(s + c - 1) * 3 + 1].w = BitsAsFloat(1u);  // last primitive of the leaf
+      }
+      depth = std::max(depth, dep[i]);
+    }
+  }
+  const bool root_leaf = d->node_count[0] != 0;
+  const uint32_t n_dev = root_leaf ? 1 : n_internal;
+  std::vector<float4> nodes(size_t(n_dev) * 4);
+  auto child_ref = [&](int32_t c) -> uint32_t {
+    if (d->node_count[c] != 0) return kLeafBit | uint32_t(d->node_start[c]);
+    return uint32_t(dev_index[c]);
+  };
+  auto child_meta = [&](int32_t c) -> uint32_t { return d->node_count[c] != 0 ? kLeafBit : 0u; };
+  auto fill = [&](uint32_t di, const float *b0, const float *b1, uint32_t r0, uint32_t r1, uint32_t m0, uint32_t m1,
+                  uint32_t axis) {
+    float4 *q = &nodes[size_t(di) * 4];
+    q[0] = make_float4(b0[0], b0[3], b0[1], b0[4]);
+    q[1] = make_float4(b1[0], b1[3], b1[1], b1[4]);
+    q[2] = make_float4(b0[2], b0[5], b1[2], b1[5]);
+    q[3] = make_float4(BitsAsFloat(r0), BitsAsFloat(r1), BitsAsFloat(m0 | (axis << 29)),
+                       BitsAsFloat(m1));
+  };
+  if (root_leaf) {
+    const float *b = d->node_bounds;
+    fill(0, b, b, kLeafBit | uint32_t(d->node_start[0]), kLeafBit | np, kLeafBit, kLeafBit, 0);
+    depth = 2;
+  } else {
+    for (uint32_t i = 0; i < nn; ++i) {
+      if (d->node_count[i] != 0) continue;
+      const int32_t l = d->node_left[i], r = d->node_right[i];
+      fill(uint32_t(dev_index[i]), d->node_bounds + size_t(l) * 6, d->node_bounds + size_t(r) * 6, child_ref(l),
+           child_ref(r), child_meta(l), child_meta(r), uint32_t(d->node_axis[i]) & 3u);
+    }
+  }
+
+  std::vector<float> mats(size_t(d->num_materials) * kMatStride);
+  for (uint32_t i = 0; i < d->num_materials; ++i) {
+    std::memcpy(&mats[size_t(i) * 16], d->materials + size_t(i) * 16, 16 * sizeof(float));
+    mats[size_t(i) * 16 + 15] = Reflectiveness(&mats[size_t(i) * 16]);
+  }
+  std::vector<float> lights(size_t(std::max(1u, d->num_lights)) * kLightStride, 0.0f);
+  uint32_t nquad = 0;
+  for (uint32_t i = 0; i < d->num_lights; ++i) {
+    std::memcpy(&lights[size_t(i) * 16], d->lights + size_t(i) * 16, 16 * sizeof(float));
+    if (int(lights[size_t(i) * 16]) == MUONX_LIGHT_QUAD) {
+      ++nquad;
+      // slot 15: |e0 x e1| (muon/integration.cc:383-384)
+      const float *L = &lights[size_t(i) * 16];
+      V3 e0{L[7], L[8], L[9]}, e1{L[10], L[11], L[12]};
+      lights[size_t(i) * 16 + 15] = length(cross(e0, e1));
+    }
+  }
+  std::vector<float> xf(d->transforms, d->transforms + size_t(d->num_transforms) * 48);
+
+  auto *s = new muonx_scene();
+  s->ctx = ctx;
+  uint64_t total = 0;
+  int rc = MUONX_OK;
+  if ((rc = Upload(s->nodes, nodes, total)) || (rc = Upload(s->prims, prims, total)) ||
+      (rc = Upload(s->xforms, xf, total)) || (rc = Upload(s->prim_id, pid, total)) ||
+      (rc = Upload(s->prim_mat, pmat, total)) || (rc = Upload(s->prim_light, plight, total)) ||
+      (rc = Upload(s->materials, mats, total)) || (rc = Upload(s->lights, lights, total))) {
+    delete s;
+    return rc;
+  }
+  if (any_vn) {
+    if ((rc = Upload(s->prim_vn, pvn, total)) || (rc = Upload(s->prim_use_vn, puvn, total))) {
+      delete s;
+      return rc;
+    }
+  }
+  DevSceneView &v = s->view;
+  v.nodes = s->nodes.as<float4>();
+  v.prims = s->prims.as<float4>();
+  v.xforms = s->xforms.as<float>();
+  v.prim_id = s->prim_id.as<int32_t>();
+  v.prim_mat = s->prim_mat.as<int32_t>();
+  v.prim_light = s->prim_light.as<int32_t>();
+  v.prim_vn = any_vn ? s->prim_vn.as<float>() : nullptr;
+  v.prim_use_vn = any_vn ? s->prim_use_vn.as<uint8_t>() : nullptr;
+  v.materials = s->materials.as<float>();
+  v.lights = s->lights.as<float>();
+  v.num_prims = np;
+  v.num_nodes = n_dev;
+  v.num_lights = d->num_lights;
+  v.num_quad_lights = nquad;
+  v.bvh_depth = depth;
+  s->bvh_depth = depth;
+  s->device_bytes = total;
+  s->host_lights = std::move(lights);
+  s->host_materials = std::move(mats);
+  *out = s;
+  return MUONX_OK;
+}
+
+extern "C" void muonx_scene_destroy(muonx_scene *s) {
+  if (!s) return;
+  if (s->ctx) (void)hipSetDevice(s->ctx->device);
+  delete s;
+}
+
+extern "C" int muonx_scene_get_info(const muonx_scene *s, muonx_scene_info *out) {
+  if (!s || !out) {
+    SetError("muonx_scene_get_info: null argument");
+    return MUONX_ERR_INVALID;
+  }
+  out->device_bytes = s->device_bytes;
+  out->num_prims = s->view.num_prims;
+  out->num_nodes = s->view.num_nodes;
+  out->bvh_depth = s->bvh_depth;
+  out->num_lights = s->view.num_lights;
+  out->num_quad_lights = s->view.num_quad_lights;
+  return MUONX_OK;
+}

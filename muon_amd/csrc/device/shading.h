@@ -1,0 +1,216 @@
+// Device BRDFs, hemisphere sampling, lights and MIS helpers, restating
+// muon/materials.cc, muon/hemisphere_sampling.cc, muon/lighting.cc and the
+// sampling helpers of muon/integration.cc with the same formulas and
+// operation order. Transcendentals use the device math library, so these
+// stages match the reference statistically rather than bit for bit.
+#ifndef MUONX_DEVICE_SHADING_H_
+#define MUONX_DEVICE_SHADING_H_
+
+#include "../common/vecmath.h"
+#include "dev_scene.h"
+#include "rng.h"
+
+namespace muonx {
+
+// muon/transform.cc:5-19
+__device__ __forceinline__ V3 rotate_to_frame(V3 s, V3 w) {
+  const V3 a = fabsf(w.y) <= 0.9f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+  const V3 u = normalize(cross(a, w));
+  const V3 v = normalize(cross(w, u));
+  return (s.x * u + s.y * v) + s.z * w;
+}
+
+__device__ __forceinline__ V3 spherical(float theta, float phi) {
+  float sp, cp, st, ct;
+  sincosf(phi, &sp, &cp);
+  sincosf(theta, &st, &ct);
+  return V3{cp * st, sp * st, ct};
+}
+
+// muon/hemisphere_sampling.cc:8-22
+__device__ __forceinline__ V3 sample_hemisphere(V3 n, uint32_t key, uint32_t &ctr) {
+  const float r1 = rng_uniform(key, ctr);
+  const float r2 = rng_uniform(key, ctr);
+  return rotate_to_frame(spherical(acosf(r1), 2.0f * kPi * r2), n);
+}
+
+// muon/hemisphere_sampling.cc:24-38
+__device__ __forceinline__ V3 sample_cosine(V3 n, uint32_t key, uint32_t &ctr) {
+  const float r1 = rng_uniform(key, ctr);
+  const float r2 = rng_uniform(key, ctr);
+  return rotate_to_frame(spherical(acosf(sqrtf(r1)), 2.0f * kPi * r2), n);
+}
+
+struct Mat {
+  V3 diffuse, specular, emission;
+  float shininess, roughness, refl;
+  int brdf;
+};
+
+__device__ __forceinline__ Mat load_mat(const float *m) {
+  Mat r;
+  r.diffuse = V3{m[3], m[4], m[5]};
+  r.specular = V3{m[6], m[7], m[8]};
+  r.emission = V3{m[9], m[10], m[11]};
+  r.shininess = m[12];
+  r.roughness = m[13];
+  r.brdf = int(m[14]);
+  r.refl = m[15];
+  return r;
+}
+
+// muon/materials.cc:131-141
+__device__ __forceinline__ float ggx_d(V3 h, V3 n, float rough) {
+  const float ha = acosf(gclamp(dot(h, n), 0.0f, 1.0f));
+  const float a2 = rough * rough;
+  const float c = cosf(ha), t = tanf(ha);
+  const float c2 = c * c;
+  const float tt = a2 + t * t;
+  const float denom = kPi * (c2 * c2) * (tt * tt);
+  return denom > 0.0f ? a2 / denom : 0.0f;
+}
+
+// muon/materials.cc:148-158
+__device__ __forceinline__ float smith_g1(V3 v, V3 n, float rough) {
+  const float vn = dot(v, n);
+  if (vn <= 0.0f) return 0.0f;
+  const float ang = acosf(gclamp(vn, 0.0f, 1.0f));
+  const float a2 = rough * rough;
+  const float t = tanf(ang);
+  return 2.0f / (1.0f + sqrtf(1.0f + a2 * (t * t)));
+}
+
+// BRDF::Sample: Lambertian muon/materials.cc:17-24, Phong :49-71, GGX :180-205.
+__device__ __forceinline__ V3 brdf_sample(const Mat &m, V3 rd, V3 n, uint32_t key, uint32_t &ctr) {
+  if (m.brdf == MUONX_BRDF_LAMBERTIAN) return sample_cosine(n, key, ctr);
+  const float r0 = rng_uniform(key, ctr);
+  if (r0 > m.refl) return sample_cosine(n, key, ctr);
+  const float r1 = rng_uniform(key, ctr);
+  const float r2 = rng_uniform(key, ctr);
+  if (m.brdf == MUONX_BRDF_PHONG) {
+    const float theta = acosf(powf(r1, 1.0f / (m.shininess + 1.0f)));
+    return rotate_to_frame(spherical(theta, 2.0f * kPi * r2), reflect(rd, n));
+  }
+  const float theta = atanf(m.roughness * sqrtf(r1) / sqrtf(1.0f - r1));
+  const V3 h = rotate_to_frame(spherical(theta, 2.0f * kPi * r2), n);
+  return reflect(rd, h);
+}
+
+// BRDF::PDF: muon/materials.cc:26-34, :73-91, :207-225.
+__device__ __forceinline__ float brdf_pdf(const Mat &m, V3 wi, V3 rd, V3 n) {
+  const float cos_term = gmax(dot(n, wi), 0.0f) * kOneOverPi;
+  if (m.brdf == MUONX_BRDF_LAMBERTIAN) return cos_term;
+  const float t = m.refl;
+  if (m.brdf == MUONX_BRDF_PHONG) {
+    return (1.0f - t) * cos_term +
+           t * (m.shininess + 1.0f) * 0.5f * kOneOverPi * powf(gmax(dot(reflect(rd, n), wi), 0.0f), m.shininess);
+  }
+  const V3 h = normalize(wi - rd);
+  return (1.0f - t) * cos_term + t * ggx_d(h, n, m.roughness) * gmax(dot(n, h), 0.0f) / (4.0f * dot(h, wi));
+}
+
+// BRDF::Eval: muon/materials.cc:36-43, :93-111, :227-266.
+__device__ __forceinline__ V3 brdf_eval(const Mat &m, V3 wi, V3 rd, V3 n) {
+  const V3 kd = m.diffuse * kOneOverPi;
+  if (m.brdf == MUONX_BRDF_LAMBERTIAN) return kd;
+  if (m.brdf == MUONX_BRDF_PHONG) {
+    return kd + m.specular * (m.shininess + 2.0f) * (0.5f * kOneOverPi) *
+                    powf(gmax(dot(reflect(rd, n), wi), 0.0f), m.shininess);
+  }
+  const V3 h = normalize(wi - rd);
+  const float one_m = 1.0f - dot(wi, h);
+  const float one_m2 = one_m * one_m;
+  const float f5 = one_m2 * one_m2 * one_m;
+  const V3 fres = m.specular + (1.0f - m.specular) * f5;
+  const float g = smith_g1(wi, n, m.roughness) * smith_g1(-rd, n, m.roughness);
+  const float d = ggx_d(h, n, m.roughness);
+  return kd + (fres * g * d) / (4.0f * dot(wi, n) * dot(-rd, n));
+}
+
+// Light record accessors (muonx_scene_desc light layout).
+struct QuadLight {
+  V3 color, corner, e0, e1;
+  float area;
+};
+__device__ __forceinline__ QuadLight load_quad(const float *L) {
+  QuadLight q;
+  q.color = V3{L[1], L[2], L[3]};
+  q.corner = V3{L[4], L[5], L[6]};
+  q.e0 = V3{L[7], L[8], L[9]};
+  q.e1 = V3{L[10], L[11], L[12]};
+  q.area = L[15];
+  return q;
+}
+
+// Quad::Intersect (muon/lighting.cc:7-36); returns the distance from the
+// origin to the hit point, or -1.
+__device__ __forceinline__ float quad_hit(const QuadLight &q, V3 o, V3 d) {
+  const V3 nrm = normalize(cross(q.e1, q.e0));
+  const float rdn = dot(d, nrm);
+  if (rdn == 0.0f) return -1.0f;
+  const float t = dot(q.corner - o, nrm) / rdn;
+  if (t < 0.0f) return -1.0f;
+  const V3 p = o + d * t;
+  const V3 pe = p - q.corner;
+  const float l0 = length(q.e0), l1 = length(q.e1);
+  const float u = dot(pe, q.e0) / l0;
+  const float v = dot(pe, q.e1) / l1;
+  if (u > 0.0f && u < l0 && v > 0.0f && v < l1) return length(p - o);
+  return -1.0f;
+}
+
+// PathTracer::NEEPDF (muon/integration.cc:564-608): area-measure pdf of the
+// ray's direction summed over the quad lights it hits, averaged over all
+// quad lights.
+__device__ __forceinline__ float nee_pdf(const DevSceneView &s, V3 o, V3 d) {
+  float pdf = 0.0f;
+  int affecting = 0;
+  for (uint32_t l = 0; l < s.num_lights; ++l) {
+    const float *L = s.lights + size_t(l) * kLightStride;
+    if (int(L[0]) != MUONX_LIGHT_QUAD) continue;
+    ++affecting;
+    const QuadLight q = load_quad(L);
+    const float dist = quad_hit(q, o, d);
+    if (dist < 0.0f) continue;
+    const V3 rn = normalize(cross(q.e0, q.e1));
+    const float cosl = fabsf(dot(rn, d));
+    const float denom = q.area * cosl;
+    if (denom == 0.0f) continue;
+    pdf += (dist * dist) / denom;
+  }
+  if (affecting == 0) return 0.0f;
+  return pdf / float(affecting);
+}
+
+// muon/integration.cc:332-338
+__device__ __forceinline__ float power_heuristic(float p0, float p1) {
+  const float denom = p0 * p0 + p1 * p1;
+  if (denom == 0.0f) return 0.0f;
+  return (p0 * p0) / denom;
+}
+
+// PathTracer::ImportanceSamplingPDF (muon/integration.cc:610-630)
+__device__ __forceinline__ float is_pdf(int mode, const Mat &m, V3 wi, V3 rd, V3 n) {
+  if (mode == MUONX_IS_HEMISPHERE) return 1.0f / (2.0f * kPi);
+  if (mode == MUONX_IS_COSINE) return gmax(dot(n, wi), 0.0f) / kPi;
+  return brdf_pdf(m, wi, rd, n);
+}
+
+// PathTracer::SampleReflection (muon/integration.cc:503-562).
+__device__ __forceinline__ bool sample_reflection(int mode, const Mat &m, V3 rd, V3 n, V3 T, uint32_t key,
+                                                  uint32_t &ctr, V3 &dir, float &pdf, V3 &next_T) {
+  if (mode == MUONX_IS_HEMISPHERE) dir = sample_hemisphere(n, key, ctr);
+  else if (mode == MUONX_IS_COSINE) dir = sample_cosine(n, key, ctr);
+  else dir = brdf_sample(m, rd, n, key, ctr);
+  if (dot(dir, n) <= 0.0f) return false;
+  const V3 f = brdf_eval(m, dir, rd, n);
+  pdf = is_pdf(mode, m, dir, rd, n);
+  if (mode == MUONX_IS_HEMISPHERE) next_T = T * 2.0f * kPi * f * gmax(dot(n, dir), 0.0f);
+  else if (mode == MUONX_IS_COSINE) next_T = T * kPi * f;
+  else next_T = T * f / pdf * gmax(dot(n, dir), 0.0f);
+  return true;
+}
+
+}  // namespace muonx
+
+#endif

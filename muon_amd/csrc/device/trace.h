@@ -1,0 +1,216 @@
+// BVH traversal for gfx950: one ray per lane, child-boxes-in-parent binary
+// nodes, a per-lane short stack in LDS (entries = child ref + entry distance),
+// and the reference's primitive tests in object space.
+//
+// Parity contract with the reference (muon/acceleration.cc:65-202,
+// muon/objects.cc:29-270):
+//  * primitive tests are op-for-op the reference's (object-space ray through
+//    M^-1 with a re-normalised direction, unnormalised-normal 1e-4 parallel
+//    test, edge tests, world distance |M p - o|), compiled without FMA
+//    contraction and with IEEE division/sqrt, so distances carry its bits;
+//  * acceptance is strict `0 < dist < best`, and children are visited in the
+//    reference's DFS order (children[dir[axis] < 0] first), so ties resolve
+//    the same way;
+//  * box tests multiply by a per-ray reciprocal instead of dividing, and
+//    widen the [t_lo, t_hi] interval by 2^-20 relative so that every box the
+//    reference's exact slab test (muon/bounds.cc:88-125) accepts is accepted
+//    here. Visiting a superset of boxes cannot change the closest hit except
+//    for distance ties within rounding.
+#ifndef MUONX_DEVICE_TRACE_H_
+#define MUONX_DEVICE_TRACE_H_
+
+#include "../common/vecmath.h"
+#include "dev_scene.h"
+
+namespace muonx {
+
+constexpr float kWidenLo = 1.0f - 1.0f / 1048576.0f;
+constexpr float kWidenHi = 1.0f + 1.0f / 1048576.0f;
+
+struct TraceRay {
+  V3 o, d, inv;
+  uint32_t dneg;  // bit a set when d[a] < 0
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+  // Axes with a zero direction component are skipped by the reference; a
+  // huge reciprocal gives the same outcome for origins inside the slab.
+  float r = 1.0f / d;
+  if (!(fabsf(r) < 1e30f)) r = copysignf(1e30f, d);
+  return r;
+}
+
+__device__ __forceinline__ TraceRay make_ray(V3 o, V3 d) {
+  TraceRay r;
+  r.o = o;
+  r.d = d;
+  r.inv = V3{safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+  r.dneg = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+  return r;
+}
+
+// Conservative slab test; returns the widened entry distance in `lo`.
+__device__ __forceinline__ bool box_hit(const TraceRay &r, float mnx, float mxx, float mny, float mxy, float mnz,
+                                        float mxz, float best, float &lo) {
+  float ax = (mnx - r.o.x) * r.inv.x, bx = (mxx - r.o.x) * r.inv.x;
+  float ay = (mny - r.o.y) * r.inv.y, by = (mxy - r.o.y) * r.inv.y;
+  float az = (mnz - r.o.z) * r.inv.z, bz = (mxz - r.o.z) * r.inv.z;
+  float l = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+  float h = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+  l = l * kWidenLo;
+  h = h * kWidenHi;
+  lo = l;
+  return l <= h && h > 0.0f && l < best;
+}
+
+// muon/objects.cc:58-175 in object space. On a hit returns the object-space
+// point; optionally the unnormalised barycentric numerators (u, v, w) and n2.
+__device__ __forceinline__ bool tri_test(V3 a, V3 b, V3 c, V3 o, V3 d, V3 &p) {
+  V3 n = cross(b - a, c - a);
+  float dn = dot(n, d);
+  if (dn > -kEps && dn < kEps) return false;
+  float t = (dot(a, n) - dot(o, n)) / dn;
+  if (t < 0.0f) return false;
+  p = o + d * t;
+  if (dot(n, cross(b - a, p - a)) < 0.0f) return false;
+  if (dot(n, cross(c - b, p - b)) < 0.0f) return false;
+  if (dot(n, cross(a - c, p - c)) < 0.0f) return false;
+  return true;
+}
+
+// muon/objects.cc:195-270 in object space (direction assumed unit length).
+__device__ __forceinline__ bool sphere_test(V3 c, float rad, V3 o, V3 d, V3 &p) {
+  V3 oc = o - c;
+  float bp = dot(d, oc);
+  float cc = dot(oc, oc) - rad * rad;
+  float disc = bp * bp - cc;
+  if (disc < 0.0f) return false;
+  float sd = sqrtf(disc);
+  float r0 = -bp + sd, r1 = -bp - sd;
+  if (r0 < 0.0f && r1 < 0.0f) return false;
+  float t = gmin(r0, r1);
+  if (t < 0.0f) t = gmax(r0, r1);
+  p = o + d * t;
+  return true;
+}
+
+// Ray.Transform(M^-1) of muon/ray.cc:7-11.
+__device__ __forceinline__ void to_object(const float *xf, V3 o, V3 d, V3 &oo, V3 &od) {
+  oo = xform_point(xf + 16, o);
+  od = normalize(xform_vec(xf + 16, d));
+}
+
+struct TraceCounters {
+  uint32_t nodes;  // internal nodes fetched (two child boxes each)
+  uint32_t prims;  // primitive tests
+};
+
+// Tests leaf-order primitive i against the ray; returns the world distance
+// on an object-space hit.
+struct ObjCache {
+  int32_t tid;
+  V3 o, d;
+};
+
+__device__ __forceinline__ bool prim_hit(const DevSceneView &s, uint32_t i, const TraceRay &r, ObjCache &oc,
+                                         float &dist, uint32_t &flags) {
+  const float4 q0 = s.prims[3 * i];
+  const float4 q1 = s.prims[3 * i + 1];
+  const uint32_t tw = __float_as_uint(q0.w);
+  flags = __float_as_uint(q1.w);
+  const int32_t tid = int32_t(tw & 0x7fffffffu);
+  const float *xf = s.xforms + size_t(tid) * 48;
+  if (tid != oc.tid) {
+    to_object(xf, r.o, r.d, oc.o, oc.d);
+    oc.tid = tid;
+  }
+  V3 p;
+  bool hit;
+  if (tw & kSphereBit) {
+    hit = sphere_test(V3{q0.x, q0.y, q0.z}, q1.x, oc.o, oc.d, p);
+  } else {
+    const float4 q2 = s.prims[3 * i + 2];
+    hit = tri_test(V3{q0.x, q0.y, q0.z}, V3{q1.x, q1.y, q1.z}, V3{q2.x, q2.y, q2.z}, oc.o, oc.d, p);
+  }
+  if (!hit) return false;
+  V3 pos = xform_point(xf, p);
+  dist = length(pos - r.o);
+  return true;
+}
+
+// Traverses the BVH. ANY: stop at the first primitive with 0 < dist < tmax
+// (HasIntersection); otherwise closest hit (Intersect). Returns the leaf-order
+// primitive index or -1; `best` holds the distance on return.
+// The stack lives at stack[k * stride] for k = 0..STACK-1.
+template <int STACK, bool ANY, bool COUNT>
+__device__ __forceinline__ int32_t trace_bvh(const DevSceneView &s, const TraceRay &r, float &best, uint2 *stack,
+                                             int stride, TraceCounters &cnt) {
+  int32_t hit_prim = -1;
+  ObjCache oc;
+  oc.tid = -1;
+  uint32_t ref = 0;  // root: internal node 0
+  int sp = 0;
+  while (true) {
+    if (!(ref & kLeafBit)) {
+      const float4 *np = s.nodes + 4 * size_t(ref);
+      const float4 n0 = np[0], n1 = np[1], n2 = np[2];
+      const int4 n3 = *reinterpret_cast<const int4 *>(np + 3);
+      if (COUNT) cnt.nodes++;
+      float lo0, lo1;
+      const bool h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
+      const bool h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+      const uint32_t axis = (uint32_t(n3.z) >> 29) & 3u;
+      const bool swap = (r.dneg >> axis) & 1u;  // near child is children[1]
+      const uint32_t near_ref = swap ? uint32_t(n3.y) : uint32_t(n3.x);
+      const uint32_t far_ref = swap ? uint32_t(n3.x) : uint32_t(n3.y);
+      const bool hn = swap ? h1 : h0;
+      const bool hf = swap ? h0 : h1;
+      const float lof = swap ? lo0 : lo1;
+      if (hn) {
+        if (hf) {
+          if (sp < STACK) {
+            stack[sp * stride] = make_uint2(far_ref, __float_as_uint(lof));
+            ++sp;
+          }
+        }
+        ref = near_ref;
+        continue;
+      }
+      if (hf) {
+        ref = far_ref;
+        continue;
+      }
+    } else {
+      uint32_t i = ref & ~kLeafBit;
+      while (true) {
+        float dist;
+        uint32_t flags;
+        if (COUNT) cnt.prims++;
+        if (prim_hit(s, i, r, oc, dist, flags) && dist > 0.0f && dist < best) {
+          best = dist;
+          hit_prim = int32_t(i);
+          if (ANY) return hit_prim;
+        }
+        if (flags & 1u) break;
+        ++i;
+      }
+    }
+    // pop, skipping entries whose box starts beyond the current best
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      const uint2 e = stack[sp * stride];
+      if (__uint_as_float(e.y) < best) {
+        ref = e.x;
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  return hit_prim;
+}
+
+}  // namespace muonx
+
+#endif

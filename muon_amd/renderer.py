@@ -1,0 +1,117 @@
+"""Host-side mirror of muon's render surface (muon/renderer.h:11-24,
+muon/options.h:11-23, muon/film.h:18-47, muon/stats.h:11-64) over the
+C ABI. Renderer(scene_file, options).Render() parses the scene, uploads it
+to the GPU, renders it there and writes the PNG, as muon::Renderer does."""
+import ctypes as C
+import dataclasses
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .scene import Context, DeviceScene, HostScene
+
+ACCELERATION = {"linear": 0, "bvh": 1}
+PARTITION = {"uniform": 0, "midpoint": 1, "sah": 2}
+
+
+@dataclasses.dataclass
+class Options:
+    """muon::Options: output overrides the scene's output; parallelism only
+    applies to host-side work (the render runs on the GPU)."""
+    output: str = ""
+    acceleration: str = "bvh"
+    partition_strategy: str = "sah"
+    parallelism: int = 1
+    show_stats: bool = True
+    default_seed: int = -1
+    device: int = 0
+
+
+@dataclasses.dataclass
+class TraceStats:
+    primary_rays: int = 0
+    secondary_rays: int = 0
+    shadow_rays: int = 0
+    render_seconds: float = 0.0
+    build_seconds: float = 0.0
+
+    def __str__(self):
+        lines = [
+            f"{'Build time':18s} : {self.build_seconds:12.2f} (sec)",
+            f"{'Render time':18s} : {self.render_seconds:12.2f} (sec)",
+            f"{'Primary rays':18s} : {self.primary_rays:12d}",
+            f"{'Secondary rays':18s} : {self.secondary_rays:12d}",
+            f"{'Shadow rays':18s} : {self.shadow_rays:12d}",
+        ]
+        return "\n".join(lines)
+
+
+class Film:
+    """Linear float accumulator [H, W, 3] (sum over samples) and its 8-bit
+    output (muon/film.cc:21-43)."""
+
+    def __init__(self, accum, spp, gamma):
+        self.accum = np.ascontiguousarray(accum, dtype=np.float32)
+        self.spp = spp
+        self.gamma = gamma
+
+    @property
+    def height(self):
+        return self.accum.shape[0]
+
+    @property
+    def width(self):
+        return self.accum.shape[1]
+
+    def rgb8(self):
+        out = np.zeros(self.accum.shape, np.uint8)
+        check(lib.muonx_film_to_rgb8(self.accum.ctypes.data, self.width, self.height, self.spp, self.gamma,
+                                     out.ctypes.data))
+        return out
+
+    def write_png(self, path):
+        img = self.rgb8()
+        check(lib.muonx_write_png(path.encode(), img.ctypes.data, self.width, self.height))
+
+
+def render_device(dev_scene, params, accum=None):
+    """Renders on the GPU; returns (accum torch tensor [H,W,3], TraceStats)."""
+    import torch
+    H, W = params.height, params.width
+    if accum is None:
+        accum = torch.empty((H, W, 3), dtype=torch.float32, device=f"cuda:{dev_scene.ctx.device}")
+    st = _lib.TraceStatsC()
+    s = torch.cuda.current_stream(accum.device).cuda_stream
+    check(lib.muonx_render(dev_scene.h, C.byref(params), accum.data_ptr(), C.byref(st), s))
+    stats = TraceStats(st.primary_rays, st.secondary_rays, st.shadow_rays, st.render_ns * 1e-9)
+    return accum, stats
+
+
+class Renderer:
+    """muon::Renderer(scene_file, options).Render()."""
+
+    def __init__(self, scene_file, options=None):
+        self.scene_file = scene_file
+        self.options = options or Options()
+
+    def Render(self):
+        o = self.options
+        t0 = time.perf_counter()
+        hs = HostScene.from_file(self.scene_file, ACCELERATION[o.acceleration], PARTITION[o.partition_strategy],
+                                 o.default_seed)
+        ctx = Context(o.device)
+        ds = DeviceScene(ctx, hs)
+        t1 = time.perf_counter()
+        accum, stats = render_device(ds, hs.params)
+        film = Film(accum.cpu().numpy(), hs.params.spp, hs.params.gamma)
+        stats.build_seconds = t1 - t0
+        output = o.output or None
+        if output:
+            film.write_png(output)
+        if o.show_stats:
+            print(stats, file=sys.stderr)
+        self.film, self.stats = film, stats
+        return film

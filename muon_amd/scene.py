@@ -1,0 +1,135 @@
+"""Host scene ingest and device-resident scenes over the C ABI."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _view(ptr, count, dtype):
+    if not ptr or count == 0:
+        return np.zeros(0, dtype)
+    ctype = {np.float32: C.c_float, np.int32: C.c_int32, np.uint8: C.c_uint8}[dtype]
+    arr = (ctype * count).from_address(ptr)
+    return np.ctypeslib.as_array(arr).copy()
+
+
+class HostScene:
+    """A parsed .muon scene with its host-built SAH BVH
+    (muon::Parser::Parse + acceleration::BVH::Init)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+        self.desc = _lib.SceneDesc()
+        check(lib.muonx_host_scene_desc(self.h, C.byref(self.desc)))
+        self.params = _lib.RenderParams()
+        check(lib.muonx_host_scene_params(self.h, C.byref(self.params)))
+
+    @classmethod
+    def from_file(cls, path, acceleration=1, partition_strategy=2, default_seed=-1):
+        opts = _lib.Options(acceleration, partition_strategy, 1, 1, default_seed)
+        out = C.c_void_p()
+        check(lib.muonx_host_scene_parse_file(path.encode(), C.byref(opts), C.byref(out)))
+        return cls(out.value)
+
+    @classmethod
+    def from_text(cls, text, acceleration=1, partition_strategy=2, default_seed=-1):
+        opts = _lib.Options(acceleration, partition_strategy, 1, 1, default_seed)
+        out = C.c_void_p()
+        b = text.encode()
+        check(lib.muonx_host_scene_parse_text(b, len(b), C.byref(opts), C.byref(out)))
+        return cls(out.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.muonx_host_scene_destroy(self.h)
+            self.h = None
+
+    @property
+    def warnings(self):
+        return lib.muonx_host_scene_warnings(self.h)
+
+    def arrays(self):
+        """Copies of the descriptor arrays (for tests and tools)."""
+        d = self.desc
+        n, nn = d.num_prims, d.num_nodes
+        return dict(
+            prim_type=_view(d.prim_type, n, np.uint8),
+            prim_geom=_view(d.prim_geom, 12 * n, np.float32).reshape(n, 12),
+            prim_vnormal=_view(d.prim_vnormal, 9 * n, np.float32).reshape(n, 9),
+            prim_use_vnormal=_view(d.prim_use_vnormal, n, np.uint8),
+            prim_transform=_view(d.prim_transform, n, np.int32),
+            prim_material=_view(d.prim_material, n, np.int32),
+            prim_light=_view(d.prim_light, n, np.int32),
+            transforms=_view(d.transforms, 48 * d.num_transforms, np.float32).reshape(-1, 3, 4, 4),
+            materials=_view(d.materials, 16 * d.num_materials, np.float32).reshape(-1, 16),
+            lights=_view(d.lights, 16 * d.num_lights, np.float32).reshape(-1, 16),
+            node_bounds=_view(d.node_bounds, 6 * nn, np.float32).reshape(nn, 6),
+            node_left=_view(d.node_left, nn, np.int32),
+            node_right=_view(d.node_right, nn, np.int32),
+            node_axis=_view(d.node_axis, nn, np.int32),
+            node_start=_view(d.node_start, nn, np.int32),
+            node_count=_view(d.node_count, nn, np.int32),
+            leaf_prims=_view(d.leaf_prims, n, np.int32),
+        )
+
+
+class Context:
+    """One device context (muonx_ctx)."""
+
+    def __init__(self, device=0):
+        out = C.c_void_p()
+        check(lib.muonx_ctx_create(device, C.byref(out)))
+        self.h = out
+        self.device = device
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.muonx_ctx_destroy(self.h)
+            self.h = None
+
+
+class DeviceScene:
+    """A scene resident in HBM (muonx_scene)."""
+
+    def __init__(self, ctx, host_scene):
+        out = C.c_void_p()
+        check(lib.muonx_scene_create(ctx.h, C.byref(host_scene.desc), C.byref(out)))
+        self.h = out
+        self.ctx = ctx  # keeps the context alive
+        self.params = host_scene.params
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.muonx_scene_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        i = _lib.SceneInfo()
+        check(lib.muonx_scene_get_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    # Device-pointer queries. o, d: torch float32 (n,3) on the GPU.
+    def intersect_closest(self, o, d, tmax=None, prim=None, t=None, stream=None):
+        import torch
+        n = o.shape[0]
+        if prim is None:
+            prim = torch.empty(n, dtype=torch.int32, device=o.device)
+        if t is None:
+            t = torch.empty(n, dtype=torch.float32, device=o.device)
+        s = torch.cuda.current_stream(o.device).cuda_stream if stream is None else stream
+        check(lib.muonx_intersect_closest(self.h, o.data_ptr(), d.data_ptr(),
+                                          None if tmax is None else tmax.data_ptr(), n, prim.data_ptr(),
+                                          t.data_ptr(), s))
+        return prim, t
+
+    def intersect_any(self, o, d, tmax=None, occ=None, stream=None):
+        import torch
+        n = o.shape[0]
+        if occ is None:
+            occ = torch.empty(n, dtype=torch.uint8, device=o.device)
+        s = torch.cuda.current_stream(o.device).cuda_stream if stream is None else stream
+        check(lib.muonx_intersect_any(self.h, o.data_ptr(), d.data_ptr(),
+                                      None if tmax is None else tmax.data_ptr(), n, occ.data_ptr(), s))
+        return occ
