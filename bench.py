@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 256 MiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -56,7 +58,6 @@ def main():
     import torch
 
     from muon_amd import Context, DeviceScene, HostScene, scenes
-    from muon_amd import _lib
     from muon_amd.renderer import render_device
 
     dev = torch.device("cuda:0")
@@ -74,6 +75,13 @@ def main():
     t = torch.empty(args.rays, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev)
 
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log(f"bench: warmup {args.warmup}")
     for _ in range(args.warmup):
         ds.intersect_closest(o, d, None, prim, t)
@@ -96,11 +104,8 @@ def main():
     hits = int((prim >= 0).sum().item())
 
     # device traversal counters for one launch
-    import ctypes as C
     ctr = torch.zeros(2, dtype=torch.int64, device=dev)
-    _lib.check(_lib.lib.muonx_intersect_closest_counted(ds.h, o.data_ptr(), d.data_ptr(), None, args.rays,
-                                                        prim.data_ptr(), t.data_ptr(), ctr.data_ptr(),
-                                                        stream.cuda_stream))
+    ds.query(o, d, counters=ctr, prim=prim, t=t)
     torch.cuda.synchronize()
     dev_nodes, dev_prims = [int(x) for x in ctr.cpu().tolist()]
 

@@ -183,13 +183,34 @@ int muonx_intersect_closest(muonx_scene *scene, const float *ray_o, const float 
 int muonx_intersect_any(muonx_scene *scene, const float *ray_o, const float *ray_d, const float *tmax,
                         uint32_t n, uint8_t *occluded, void *stream);
 
-/* muonx_intersect_closest plus traversal counters, the device analogue of
- * the TraceStats increments in BVH::Intersect (muon/acceleration.cc:92,101,108,114):
- * counters[0] += internal nodes fetched (two child boxes each),
- * counters[1] += primitive tests. counters: device pointer to 2 uint64. */
-int muonx_intersect_closest_counted(muonx_scene *scene, const float *ray_o, const float *ray_d, const float *tmax,
-                                    uint32_t n, int32_t *prim_id, float *t, unsigned long long *counters,
-                                    void *stream);
+/* Traversal modes.
+ * FAST: reciprocal slab test widened by 2^-20 relative, so every box the
+ *   reference's slab test accepts is accepted; identical results except for
+ *   distance ties and hits within rounding of a primitive's edge.
+ * EXACT: the reference's division slab test (muon/bounds.cc:88-125) with the
+ *   exact entry distance kept for the far child, so the traversal visits the
+ *   reference's nodes and primitives in its order: results are the
+ *   reference's, ties included. */
+#define MUONX_TRAVERSAL_FAST 0
+#define MUONX_TRAVERSAL_EXACT 1
+
+typedef struct muonx_query {
+  const float *ray_o;          /* device, 3n floats */
+  const float *ray_d;          /* device, 3n floats */
+  const float *tmax;           /* device, n floats, or NULL (+inf) */
+  uint32_t n;
+  int32_t any_hit;             /* 0 closest (Intersect), 1 any (HasIntersection) */
+  int32_t mode;                /* MUONX_TRAVERSAL_* */
+  int32_t *prim_id;            /* closest: parse-order id or -1 */
+  float *t;                    /* closest: world distance, may be NULL */
+  uint8_t *occluded;           /* any-hit: 1 if a primitive lies at 0 < d < tmax */
+  unsigned long long *counters; /* device u64[2] or NULL: += internal nodes fetched,
+                                   += primitive tests (cf. the TraceStats increments,
+                                   muon/acceleration.cc:92,108) */
+} muonx_query;
+
+/* General batched query: closest or any hit, either traversal mode. */
+int muonx_intersect(muonx_scene *scene, const muonx_query *q, void *stream);
 
 /* ------------------------------------------------------------ render
  * Replaces the tile loop of muon::Renderer::Render (muon/renderer.cc:40-76):

@@ -48,6 +48,12 @@ class TraceStatsC(C.Structure):
                 ("render_ns", C.c_uint64)]
 
 
+class Query(C.Structure):
+    _fields_ = [("ray_o", C.c_void_p), ("ray_d", C.c_void_p), ("tmax", C.c_void_p), ("n", C.c_uint32),
+                ("any_hit", C.c_int32), ("mode", C.c_int32), ("prim_id", C.c_void_p), ("t", C.c_void_p),
+                ("occluded", C.c_void_p), ("counters", C.c_void_p)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("device_bytes", C.c_uint64), ("num_prims", C.c_uint32), ("num_nodes", C.c_uint32),
                 ("bvh_depth", C.c_uint32), ("num_lights", C.c_uint32), ("num_quad_lights", C.c_uint32)]
@@ -71,7 +77,7 @@ SIGNATURES = [
     ("muonx_scene_get_info", C.c_int, [P, C.POINTER(SceneInfo)]),
     ("muonx_intersect_closest", C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
     ("muonx_intersect_any", C.c_int, [P, P, P, P, C.c_uint32, P, P]),
-    ("muonx_intersect_closest_counted", C.c_int, [P, P, P, P, C.c_uint32, P, P, P, P]),
+    ("muonx_intersect", C.c_int, [P, C.POINTER(Query), P]),
     ("muonx_render", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC), P]),
     ("muonx_render_host", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC)]),
     ("muonx_film_to_rgb8", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),
@@ -80,6 +86,14 @@ SIGNATURES = [
 
 
 def _load():
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7)
+    # but a different file name. Loading it first makes libmuonx bind to that
+    # already-loaded runtime instead of pulling a second HIP/HSA runtime into
+    # the process, which then finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(library_path):
         raise ImportError(
             f"{library_path} is missing: build the HIP library first "

@@ -9,7 +9,7 @@ namespace muonx {
 
 constexpr int kQueryBlock = 128;
 
-int LaunchQuery(const DevSceneView &s, bool any, const float *ro, const float *rd, const float *tmax, uint32_t n,
+int LaunchQuery(const DevSceneView &s, bool any, bool exact, const float *ro, const float *rd, const float *tmax, uint32_t n,
                 int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters, hipStream_t stream);
 
 }  // namespace muonx

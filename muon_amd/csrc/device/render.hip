@@ -1,6 +1,7 @@
 // muonx_render: camera ray generation + integrators on the GPU.
 // Deterministic integrators (normals / albedo / depth, muon/integration.cc:47-76)
-// run as one fused kernel; the path tracer runs as a wavefront pipeline
+// run as one fused kernel with the exact traversal mode (the reference's
+// node visits, so ties and edge hits resolve as in muon); the path tracer runs as a wavefront pipeline
 // (pathtracer.hip).
 #include <chrono>
 
@@ -51,7 +52,7 @@ __global__ void __launch_bounds__(kSimpleBlock) k_simple(DevSceneView s, CamPara
       float best = __builtin_inff();
       TraceCounters cnt;
       ++traced;
-      const int32_t p = trace_bvh<STACK, false, false>(s, r, best, stack + threadIdx.x, kSimpleBlock, cnt);
+      const int32_t p = trace_bvh<STACK, false, false, true>(s, r, best, stack + threadIdx.x, kSimpleBlock, cnt);
       V3 c{0.0f, 0.0f, 0.0f};
       if (p >= 0) {
         if (integrator == MUONX_INTEGRATOR_ALBEDO) {

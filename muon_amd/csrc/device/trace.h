@@ -63,6 +63,37 @@ __device__ __forceinline__ bool box_hit(const TraceRay &r, float mnx, float mxx,
   return l <= h && h > 0.0f && l < best;
 }
 
+// The reference's slab test (muon/bounds.cc:88-125) with its divisions and
+// zero-direction skip; returns its t_min in `lo`. With this test and the
+// exact t_min kept on the stack, the traversal makes the reference's accept
+// / reject decision for every node: the near child is tested with the same
+// best distance the reference uses when it descends, the far child's
+// distance-independent part (overlap, t_max > 0) is decided at the parent
+// and its `t_min < best` part when it is popped, as the reference does.
+// Skipping the root's own test is exact: child intervals are contained in
+// the root's, since correctly rounded arithmetic is monotonic.
+__device__ __forceinline__ bool box_hit_exact(const TraceRay &r, float mnx, float mxx, float mny, float mxy,
+                                              float mnz, float mxz, float best, float &lo) {
+  float tmin = 0.0f, tmax = __builtin_inff();
+  const float mn[3] = {mnx, mny, mnz}, mx[3] = {mxx, mxy, mxz};
+  const float o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (d[a] == 0.0f) continue;
+    float ta = (mn[a] - o[a]) / d[a];
+    float tb = (mx[a] - o[a]) / d[a];
+    if (ta > tb) {
+      const float tmp = ta;
+      ta = tb;
+      tb = tmp;
+    }
+    tmin = gmax(tmin, ta);
+    tmax = gmin(tmax, tb);
+  }
+  lo = tmin;
+  return !(tmin > tmax) && tmax > 0.0f && tmin < best;
+}
+
 // muon/objects.cc:58-175 in object space. On a hit returns the object-space
 // point; optionally the unnormalised barycentric numerators (u, v, w) and n2.
 __device__ __forceinline__ bool tri_test(V3 a, V3 b, V3 c, V3 o, V3 d, V3 &p) {
@@ -139,10 +170,13 @@ __device__ __forceinline__ bool prim_hit(const DevSceneView &s, uint32_t i, cons
 }
 
 // Traverses the BVH. ANY: stop at the first primitive with 0 < dist < tmax
-// (HasIntersection); otherwise closest hit (Intersect). Returns the leaf-order
+// (HasIntersection); otherwise closest hit (Intersect). EXACT selects the
+// reference's division slab test (identical node visits, used by the
+// deterministic integrators and the exact query mode); the default is the
+// conservative reciprocal test. Returns the leaf-order
 // primitive index or -1; `best` holds the distance on return.
 // The stack lives at stack[k * stride] for k = 0..STACK-1.
-template <int STACK, bool ANY, bool COUNT>
+template <int STACK, bool ANY, bool COUNT, bool EXACT = false>
 __device__ __forceinline__ int32_t trace_bvh(const DevSceneView &s, const TraceRay &r, float &best, uint2 *stack,
                                              int stride, TraceCounters &cnt) {
   int32_t hit_prim = -1;
@@ -157,8 +191,14 @@ __device__ __forceinline__ int32_t trace_bvh(const DevSceneView &s, const TraceR
       const int4 n3 = *reinterpret_cast<const int4 *>(np + 3);
       if (COUNT) cnt.nodes++;
       float lo0, lo1;
-      const bool h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
-      const bool h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+      bool h0, h1;
+      if (EXACT) {
+        h0 = box_hit_exact(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
+        h1 = box_hit_exact(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+      } else {
+        h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
+        h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+      }
       const uint32_t axis = (uint32_t(n3.z) >> 29) & 3u;
       const bool swap = (r.dneg >> axis) & 1u;  // near child is children[1]
       const uint32_t near_ref = swap ? uint32_t(n3.y) : uint32_t(n3.x);

@@ -111,25 +111,26 @@ class DeviceScene:
         return {f: getattr(i, f) for f, _ in i._fields_}
 
     # Device-pointer queries. o, d: torch float32 (n,3) on the GPU.
-    def intersect_closest(self, o, d, tmax=None, prim=None, t=None, stream=None):
+    def query(self, o, d, tmax=None, any_hit=False, exact=False, counters=None, stream=None, prim=None, t=None,
+              occ=None):
         import torch
         n = o.shape[0]
-        if prim is None:
-            prim = torch.empty(n, dtype=torch.int32, device=o.device)
-        if t is None:
-            t = torch.empty(n, dtype=torch.float32, device=o.device)
-        s = torch.cuda.current_stream(o.device).cuda_stream if stream is None else stream
-        check(lib.muonx_intersect_closest(self.h, o.data_ptr(), d.data_ptr(),
-                                          None if tmax is None else tmax.data_ptr(), n, prim.data_ptr(),
-                                          t.data_ptr(), s))
-        return prim, t
-
-    def intersect_any(self, o, d, tmax=None, occ=None, stream=None):
-        import torch
-        n = o.shape[0]
-        if occ is None:
+        if any_hit and occ is None:
             occ = torch.empty(n, dtype=torch.uint8, device=o.device)
+        if not any_hit:
+            if prim is None:
+                prim = torch.empty(n, dtype=torch.int32, device=o.device)
+            if t is None:
+                t = torch.empty(n, dtype=torch.float32, device=o.device)
+        q = _lib.Query(o.data_ptr(), d.data_ptr(), None if tmax is None else tmax.data_ptr(), n, int(any_hit),
+                       int(exact), None if prim is None else prim.data_ptr(), None if t is None else t.data_ptr(),
+                       None if occ is None else occ.data_ptr(), None if counters is None else counters.data_ptr())
         s = torch.cuda.current_stream(o.device).cuda_stream if stream is None else stream
-        check(lib.muonx_intersect_any(self.h, o.data_ptr(), d.data_ptr(),
-                                      None if tmax is None else tmax.data_ptr(), n, occ.data_ptr(), s))
-        return occ
+        check(lib.muonx_intersect(self.h, C.byref(q), s))
+        return occ if any_hit else (prim, t)
+
+    def intersect_closest(self, o, d, tmax=None, prim=None, t=None, stream=None, exact=False):
+        return self.query(o, d, tmax, False, exact, None, stream, prim, t)
+
+    def intersect_any(self, o, d, tmax=None, occ=None, stream=None, exact=False):
+        return self.query(o, d, tmax, True, exact, None, stream, occ=occ)

@@ -175,6 +175,35 @@ int oracle_prim_intersect(void *p, int32_t id, const float *o, const float *d, f
   return -1;
 }
 
+// Smallest normalised barycentric coordinate min(u,v,w)/|n|^2 of the ray's
+// object-space hit on triangle `id` (muon/objects.cc:113-161, before any
+// rejection), or 1 for a sphere. Values near 0 mean the ray passes within
+// rounding of the triangle's edge. Returns -1 if the ray is parallel.
+int oracle_prim_edge(void *p, int32_t id, const float *o, const float *d, float *out) {
+  Scene &s = *static_cast<Handle *>(p)->scene;
+  for (auto &pr : s.root->primitives_) {
+    if (pr->id != id) continue;
+    if (pr->is_sphere) {
+      *out = 1.0f;
+      return 0;
+    }
+    auto *t = static_cast<Tri *>(pr.get());
+    Ray r = Ray(V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2])).Transform(*pr->inv_transform);
+    float dn = dot(t->normal_, r.d);
+    if (dn == 0.0f) return -1;
+    const V3 &a = t->v0_.pos, &b = t->v1_.pos, &c = t->v2_.pos;
+    float tt = (dot(a, t->normal_) - dot(r.o, t->normal_)) / dn;
+    V3 q = r.At(tt);
+    float v = dot(t->normal_, cross(b - a, q - a));
+    float w = dot(t->normal_, cross(c - b, q - b));
+    float u = dot(t->normal_, cross(a - c, q - c));
+    float m = std::min(u, std::min(v, w));
+    *out = m / t->normal_length2_;
+    return 0;
+  }
+  return -1;
+}
+
 int64_t oracle_bvh_num_nodes(void *p) {
   Scene &s = *static_cast<Handle *>(p)->scene;
   auto *bvh = dynamic_cast<BVH *>(s.root.get());

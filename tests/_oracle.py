@@ -73,6 +73,14 @@ class Scene:
         r = self._lib.oracle_prim_intersect(self.h, int(prim_id), _ptr(o), _ptr(d), _c.byref(out))
         return float(out.value) if r == 1 else None
 
+    def prim_edge(self, prim_id, o, d):
+        """min normalised barycentric of the ray's hit on the triangle (1 for spheres)."""
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        out = _c.c_float(0.0)
+        r = self._lib.oracle_prim_edge(self.h, int(prim_id), _ptr(o), _ptr(d), _c.byref(out))
+        return float(out.value) if r == 0 else 0.0
+
     def camera(self):
         out = np.zeros(14, np.float32)
         if self._lib.oracle_camera(self.h, out.ctypes.data) != 0:
@@ -126,6 +134,7 @@ class Oracle:
         lib.oracle_intersect.argtypes = [_P, _c.c_int64, _P, _P, _P, _P, _P, _P, _P, _c.c_int]
         lib.oracle_occluded.argtypes = [_P, _c.c_int64, _P, _P, _P, _P, _P, _c.c_int]
         lib.oracle_prim_intersect.argtypes = [_P, _c.c_int32, _P, _P, _P]
+        lib.oracle_prim_edge.argtypes = [_P, _c.c_int32, _P, _P, _P]
         lib.oracle_bvh_num_nodes.restype = _c.c_int64
         lib.oracle_bvh_num_nodes.argtypes = [_P]
         lib.oracle_bvh_dump.argtypes = [_P] * 7

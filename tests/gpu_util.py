@@ -16,18 +16,21 @@ def scene_bounds(host_scene):
     return b[0, :3], b[0, 3:]
 
 
-def classify_mismatches(orc_scene, o, d, prim_gpu, t_gpu, prim_ref, t_ref, rel=1e-6):
-    """Returns the indices of closest-hit disagreements that are not distance
-    ties: for a tie, the oracle's own distance to the primitive the GPU
-    reported agrees with the oracle's best distance to `rel`."""
+def classify_mismatches(orc_scene, o, d, prim_gpu, t_gpu, prim_ref, t_ref, rel=1e-6, edge=1e-6):
+    """Indices of closest-hit disagreements outside the parity exclusions:
+    a disagreement is excluded when it is a distance tie (the oracle's own
+    distance to the primitive the GPU reported equals its best distance to
+    `rel`) or when either reported hit lies within `edge` of a triangle edge
+    (min normalised barycentric < edge)."""
     bad = []
     for i in np.nonzero(prim_gpu != prim_ref)[0]:
-        if prim_gpu[i] < 0 or prim_ref[i] < 0:
-            # a hit on one side only: allowed only for grazing hits, which the
-            # oracle confirms by a hit at the same distance on re-test
-            bad.append(int(i))
+        near_edge = any(p >= 0 and orc_scene.prim_edge(int(p), o[i], d[i]) < edge
+                        for p in (int(prim_gpu[i]), int(prim_ref[i])))
+        if near_edge:
             continue
-        dg = orc_scene.prim_distance(int(prim_gpu[i]), o[i], d[i])
-        if dg is None or abs(dg - t_ref[i]) > rel * max(abs(t_ref[i]), 1e-30):
-            bad.append(int(i))
+        if prim_gpu[i] >= 0 and prim_ref[i] >= 0:
+            dg = orc_scene.prim_distance(int(prim_gpu[i]), o[i], d[i])
+            if dg is not None and abs(dg - t_ref[i]) <= rel * max(abs(t_ref[i]), 1e-30):
+                continue
+        bad.append(int(i))
     return bad

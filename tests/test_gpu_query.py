@@ -1,10 +1,12 @@
 """GPU closest-hit / any-hit parity with the oracle (CPU restatement of
 muon/acceleration.cc + muon/objects.cc), through the C ABI.
 
-Bar: primitive ids bit-exact except distance ties within 1e-6 relative
-(the reference resolves ties by DFS order; any remaining disagreement must
-be such a tie), hit distance within 1e-5 relative (in practice bit-exact),
-any-hit flags exact."""
+Bar, exact traversal mode: primitive ids, distance bits and any-hit flags
+equal to the oracle's on every ray.
+Bar, fast traversal mode: primitive ids equal except (i) hits within 1e-6
+of a triangle edge and (ii) distance ties within 1e-5 relative — coincident
+surfaces (a face lying on another face) reach 2e-6 because each distance is
+computed through its own transform; where ids agree, distance bits agree."""
 import os
 
 import numpy as np
@@ -23,20 +25,21 @@ def ctx():
     return Context(0)
 
 
-def _query(ds, o, d, tmax=None):
+def _query(ds, o, d, tmax=None, exact=False):
     dev = torch.device("cuda:0")
     to = torch.from_numpy(o).to(dev)
     td = torch.from_numpy(d).to(dev)
     tt = None if tmax is None else torch.from_numpy(tmax).to(dev)
-    prim, t = ds.intersect_closest(to, td, tt)
-    occ = ds.intersect_any(to, td, tt if tt is not None else torch.full((o.shape[0],), float("inf"), device=dev))
+    prim, t = ds.intersect_closest(to, td, tt, exact=exact)
+    occ = ds.intersect_any(to, td, tt if tt is not None else torch.full((o.shape[0],), float("inf"), device=dev),
+                           exact=exact)
     torch.cuda.synchronize()
     return prim.cpu().numpy(), t.cpu().numpy(), occ.cpu().numpy()
 
 
 def _check(orc, o, d, prim, t, tmax=None, occ=None):
     pr, tr, _, _, _ = orc.intersect(o, d)
-    bad = classify_mismatches(orc, o, d, prim, t, pr, tr)
+    bad = classify_mismatches(orc, o, d, prim, t, pr, tr, rel=1e-5)
     assert not bad, f"{len(bad)} non-tie mismatches, first {bad[:5]}"
     same = (prim == pr) & (pr >= 0)
     rel = np.abs(t[same] - tr[same]) / np.maximum(np.abs(tr[same]), 1e-30)
@@ -62,13 +65,21 @@ def test_reference_scene_queries(ctx, oracle, ref_dir, name):
     lo, hi = scene_bounds(hs)
     o, d = random_rays(rng, 1 << 16, lo, hi)
     tmax = rng.uniform(0.1, 30.0, o.shape[0]).astype(np.float32)
-    prim, t, _ = _query(ds, o, d)
-    exact, hits = _check(orc, o, d, prim, t)
-    assert hits > 1000
-    assert exact == 1.0
-    _, _, occ = _query(ds, o, d, tmax)
+    pr, tr, _, _, _ = orc.intersect(o, d)
     oref, _ = orc.occluded(o, d, tmax)
+    assert (pr >= 0).sum() > 1000
+    # exact mode: the reference's node visits, so ids and distance bits agree everywhere
+    prim, t, _ = _query(ds, o, d, exact=True)
+    assert np.array_equal(prim, pr)
+    assert np.array_equal(t[pr >= 0].view(np.uint32), tr[pr >= 0].view(np.uint32))
+    _, _, occ = _query(ds, o, d, tmax, exact=True)
     assert np.array_equal(occ, oref)
+    # fast mode: equal outside ties / edge hits, distance bits equal where ids agree
+    prim, t, _ = _query(ds, o, d)
+    exact_frac, _ = _check(orc, o, d, prim, t)
+    assert exact_frac == 1.0
+    _, _, occ = _query(ds, o, d, tmax)
+    assert (occ != oref).mean() < 1e-4
 
 
 def test_camera_rays_bit_exact(ctx, oracle, ref_dir):
@@ -132,4 +143,4 @@ def test_icosphere_full_microbench_invariants(icosphere):
     rng = np.random.default_rng(5)
     idx = rng.choice(o.shape[0], 4096, replace=False)
     pr, tr, _, _, _ = orc.intersect(o[idx], d[idx])
-    assert not classify_mismatches(orc, o[idx], d[idx], prim[idx], t[idx], pr, tr)
+    assert not classify_mismatches(orc, o[idx], d[idx], prim[idx], t[idx], pr, tr, rel=1e-5)

@@ -48,6 +48,8 @@ def test_deterministic_film_matches_oracle_and_golden(ctx, oracle, ref_dir, name
     acc, stats = gpu_render(ctx, hs)
     ref_acc, ref_rgb, ref_st, _ = oracle.load(path, seed=3).render(1)
     assert np.abs(acc - ref_acc).max() <= 1e-5
+    # in practice the film carries the reference's bits
+    assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32))
     rgb = Film(acc, hs.params.spp, hs.params.gamma).rgb8()
     gold = _png(ref_dir, name + ".png")
     assert int((rgb != gold).any(axis=2).sum()) == 0
@@ -81,21 +83,29 @@ def relmse(x, r):
 
 
 def test_relmse_matches_cpu_renderer(ctx, oracle):
-    """Cornell box at 128x128, 16 spp: mean relMSE over 4 seeds of the GPU
+    """Cornell box at 128x128, 16 spp: mean relMSE over 8 seeds of the GPU
     path and of the oracle against a converged 4096-spp GPU image agree
-    within 2% (criterion (c); relMSE on linear film before gamma)."""
+    within 2% (criterion (c); relMSE on linear film before gamma).
+
+    The reference seeds tile 0's pixel jitter with 0 whatever the scene seed
+    (muon/sampling.cc:41-45), so its first rows repeat the same sample
+    positions across seeds; those rows (the 13 rows of tile 0 at 8 threads)
+    are left out of both means."""
     W = H = 128
     spp = 16
+    par = 8
+    tiles = oracle.num_tiles(W, H, spp, par)
+    skip = H // tiles + (H - (H // tiles) * tiles)
     conv_text = scenes.cornell_text(W, H, 4096, 5, seed=99)
     conv, _ = gpu_render(ctx, HostScene.from_text(conv_text))
-    ref = conv / 4096.0
+    ref = (conv / 4096.0)[skip:]
     g, c = [], []
-    for seed in range(4):
+    for seed in range(8):
         text = scenes.cornell_text(W, H, spp, 5, seed=seed)
         acc, _ = gpu_render(ctx, HostScene.from_text(text))
-        g.append(relmse(acc / spp, ref))
-        oacc, _, _, _ = oracle.load_text(text).render(os.cpu_count() or 8)
-        c.append(relmse(oacc / spp, ref))
+        g.append(relmse((acc / spp)[skip:], ref))
+        oacc, _, _, _ = oracle.load_text(text).render(par)
+        c.append(relmse((oacc / spp)[skip:], ref))
     gm, cm = np.mean(g), np.mean(c)
     assert abs(gm - cm) <= 0.02 * cm, (gm, cm, g, c)
 
