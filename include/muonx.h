@@ -166,6 +166,22 @@ int muonx_host_scene_warnings(const muonx_host_scene *hs);
 int muonx_ctx_create(int device, muonx_ctx **out);
 void muonx_ctx_destroy(muonx_ctx *ctx);
 
+/* Launch tuning of a context (no counterpart in the reference, whose only
+ * knob is --parallelism). Traversal stack: 0 = whole stack in LDS (capacity
+ * from the BVH depth); 8 or 16 = LDS ring of that many entries per lane with
+ * older entries spilled to a per-thread HBM region (more waves per CU).
+ * while_while: lanes walk internal nodes until each holds a leaf, then test
+ * leaves together. blocks_per_cu: persistent grid size. path_budget_mb:
+ * HBM for the path tracer's per-path state and queues. */
+typedef struct muonx_tuning {
+  int32_t stack_entries;
+  int32_t while_while;
+  int32_t blocks_per_cu;
+  int32_t path_budget_mb;
+} muonx_tuning;
+int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t);
+int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);
+
 /* Uploads a scene; replaces the Structure::Init()'d acceleration structure
  * (muon/acceleration.h:26-53) with a device-resident BVH. Immutable after
  * creation and shareable by concurrent renders. */

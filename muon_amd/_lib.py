@@ -54,6 +54,11 @@ class Query(C.Structure):
                 ("occluded", C.c_void_p), ("counters", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("stack_entries", C.c_int32), ("while_while", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("path_budget_mb", C.c_int32)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("device_bytes", C.c_uint64), ("num_prims", C.c_uint32), ("num_nodes", C.c_uint32),
                 ("bvh_depth", C.c_uint32), ("num_lights", C.c_uint32), ("num_quad_lights", C.c_uint32)]
@@ -72,6 +77,8 @@ SIGNATURES = [
     ("muonx_host_scene_warnings", C.c_int, [P]),
     ("muonx_ctx_create", C.c_int, [C.c_int, C.POINTER(P)]),
     ("muonx_ctx_destroy", None, [P]),
+    ("muonx_ctx_set_tuning", C.c_int, [P, C.POINTER(Tuning)]),
+    ("muonx_ctx_get_tuning", C.c_int, [P, C.POINTER(Tuning)]),
     ("muonx_scene_create", C.c_int, [P, C.POINTER(SceneDesc), C.POINTER(P)]),
     ("muonx_scene_destroy", None, [P]),
     ("muonx_scene_get_info", C.c_int, [P, C.POINTER(SceneInfo)]),

@@ -59,7 +59,32 @@ void muonx_ctx_destroy(muonx_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   DestroyWorkspace(ctx->ws);
+  if (ctx->spill_done) (void)hipEventDestroy(ctx->spill_done);
   delete ctx;
+}
+
+int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t) {
+  if (!ctx || !t) {
+    SetError("muonx_ctx_set_tuning: null argument");
+    return MUONX_ERR_INVALID;
+  }
+  if (!(t->stack_entries == 0 || t->stack_entries == 8 || t->stack_entries == 16) || t->blocks_per_cu < 1 ||
+      t->blocks_per_cu > 64 || t->path_budget_mb < 16) {
+    SetError("muonx_ctx_set_tuning: stack_entries must be 0, 8 or 16; blocks_per_cu 1..64; path_budget_mb >= 16");
+    return MUONX_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning = *t;
+  return MUONX_OK;
+}
+
+int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t) {
+  if (!ctx || !t) {
+    SetError("muonx_ctx_get_tuning: null argument");
+    return MUONX_ERR_INVALID;
+  }
+  *t = ctx->tuning;
+  return MUONX_OK;
 }
 
 int muonx_host_scene_parse_text(const char *text, size_t len, const muonx_options *opts, muonx_host_scene **out) {

@@ -23,6 +23,7 @@
 #include "pt_kernels.h"
 #include "render.h"
 #include "shading.h"
+#include "traverse_launch.h"
 
 namespace muonx {
 
@@ -71,9 +72,9 @@ __global__ void __launch_bounds__(kPtBlock) k_raygen(PtArgs a, PtGen g) {
 }
 
 // ---------------------------------------------------------------- extend
-template <int STACK>
-__global__ void __launch_bounds__(kPtBlock) k_extend(PtArgs a, int cur, int depth) {
-  __shared__ uint2 stack[STACK * kPtBlock];
+template <int KIND, bool WW>
+__global__ void __launch_bounds__(kPtBlock) k_extend(PtArgs a, StackArgs sa, int cur, int depth) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kPtBlock];
   const uint32_t n = a.counts[cur == 0 ? kQCur : kQNext];
   const float4 *q = a.q_ray[cur];
   uint32_t traced = 0;
@@ -84,7 +85,8 @@ __global__ void __launch_bounds__(kPtBlock) k_extend(PtArgs a, int cur, int dept
       const TraceRay r = make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z});
       float best = __builtin_inff();
       TraceCounters c;
-      a.hit_prim[i] = trace_bvh<STACK, false, false>(a.scene, r, best, stack + threadIdx.x, kPtBlock, c);
+      auto st = make_stack<KIND>(lds, kPtBlock, sa);
+      a.hit_prim[i] = trace_bvh_s<false, false, false, WW>(a.scene, r, best, st, c);
       ++traced;
     }
   }
@@ -259,9 +261,9 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int cur, int depth
 
 // ---------------------------------------------------------------- shadow (any hit)
 // Visits the K shadow slots of each shaded path.
-template <int STACK>
-__global__ void __launch_bounds__(kPtBlock) k_shadow(PtArgs a, int cur) {
-  __shared__ uint2 stack[STACK * kPtBlock];
+template <int KIND, bool WW>
+__global__ void __launch_bounds__(kPtBlock) k_shadow(PtArgs a, StackArgs sa, int cur) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kPtBlock];
   const uint32_t npaths_shaded = a.counts[cur == 0 ? kQCur : kQNext];
   const uint32_t K = uint32_t(a.shadow_per_vertex);
   const uint32_t n = npaths_shaded * K;
@@ -277,7 +279,8 @@ __global__ void __launch_bounds__(kPtBlock) k_shadow(PtArgs a, int cur) {
         const TraceRay r = make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z});
         float best = sd.w;
         TraceCounters c;
-        const int32_t hit = trace_bvh<STACK, true, false>(a.scene, r, best, stack + threadIdx.x, kPtBlock, c);
+        auto st = make_stack<KIND>(lds, kPtBlock, sa);
+        const int32_t hit = trace_bvh_s<true, false, false, WW>(a.scene, r, best, st, c);
         a.s_vis[slot] = hit < 0 ? 1 : 0;
         ++traced;
       }
@@ -287,9 +290,9 @@ __global__ void __launch_bounds__(kPtBlock) k_shadow(PtArgs a, int cur) {
 }
 
 // ---------------------------------------------------------------- MIS extend
-template <int STACK>
-__global__ void __launch_bounds__(kPtBlock) k_mis(PtArgs a) {
-  __shared__ uint2 stack[STACK * kPtBlock];
+template <int KIND, bool WW>
+__global__ void __launch_bounds__(kPtBlock) k_mis(PtArgs a, StackArgs sa) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kPtBlock];
   const uint32_t n = a.counts[kQMis];
   uint32_t traced = 0;
   for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
@@ -300,7 +303,8 @@ __global__ void __launch_bounds__(kPtBlock) k_mis(PtArgs a) {
       const TraceRay r = make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z});
       float best = __builtin_inff();
       TraceCounters c;
-      const int32_t hit = trace_bvh<STACK, false, false>(a.scene, r, best, stack + threadIdx.x, kPtBlock, c);
+      auto st = make_stack<KIND>(lds, kPtBlock, sa);
+      const int32_t hit = trace_bvh_s<false, false, false, WW>(a.scene, r, best, st, c);
       ++traced;
       if (hit >= 0) {
         const int32_t light = a.scene.prim_light[hit];
@@ -375,15 +379,24 @@ int ShadowPerVertex(const muonx_scene *scene, int strata, int per) {
   return K;
 }
 
-template <int STACK>
-void LaunchBounce(const PtArgs &a, int cur, int depth, int grid, bool any_nee, bool mis, hipStream_t st) {
-  hipLaunchKernelGGL(k_extend<STACK>, dim3(grid), dim3(kPtBlock), 0, st, a, cur, depth);
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kPtBlock), 0, st, a, cur, depth);
-  if (any_nee) hipLaunchKernelGGL(k_shadow<STACK>, dim3(grid), dim3(kPtBlock), 0, st, a, cur);
-  if (mis) hipLaunchKernelGGL(k_mis<STACK>, dim3(grid), dim3(kPtBlock), 0, st, a);
-  if (any_nee) hipLaunchKernelGGL(k_gather, dim3(grid), dim3(kPtBlock), 0, st, a, cur);
-  hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, st, a.counts, cur);
-}
+struct BounceLaunch {
+  const PtArgs &a;
+  const TraverseCfg &cfg;
+  int cur, depth;
+  bool any_nee, mis;
+  hipStream_t st;
+  template <int KIND, bool WW>
+  void run() const {
+    const StackArgs sa{cfg.spill, cfg.gcap};
+    const dim3 grid(cfg.grid), block(kPtBlock);
+    hipLaunchKernelGGL((k_extend<KIND, WW>), grid, block, 0, st, a, sa, cur, depth);
+    hipLaunchKernelGGL(k_shade, grid, block, 0, st, a, cur, depth);
+    if (any_nee) hipLaunchKernelGGL((k_shadow<KIND, WW>), grid, block, 0, st, a, sa, cur);
+    if (mis) hipLaunchKernelGGL((k_mis<KIND, WW>), grid, block, 0, st, a, sa);
+    if (any_nee) hipLaunchKernelGGL(k_gather, grid, block, 0, st, a, cur);
+    hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, st, a.counts, cur);
+  }
+};
 
 }  // namespace
 
@@ -415,7 +428,7 @@ int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *ac
   // Path capacity: bounded by a memory budget and by the job size.
   const uint64_t per_path = 4 * (3 + 3 + 1) + 2 * 2 * 16 + 4 + 3 * 4 * 2 + 2 * 16 +
                             uint64_t(a.shadow_per_vertex) * (3 * 4 + 2 * 16 + 2);
-  const uint64_t budget = uint64_t(2) << 30;
+  const uint64_t budget = uint64_t(std::max(16, scene->ctx->tuning.path_budget_mb)) << 20;
   uint64_t cap = std::max<uint64_t>(1, budget / per_path);
   cap = std::min<uint64_t>(cap, total);
   cap = std::min<uint64_t>(cap, uint64_t(1) << 26);
@@ -457,12 +470,13 @@ int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *ac
   MX_HIP(hipMemsetAsync(accum, 0, size_t(npix) * 3 * sizeof(float), stream));
   if (!any_nee) MX_HIP(hipMemsetAsync(a.s_valid, 0, P * K, stream));
 
-  int num_cus = scene->ctx->num_cus > 0 ? scene->ctx->num_cus : 256;
-  const int grid_cap = num_cus * 16;
+  TraverseCfg cfg;
+  int rc = PlanTraverse(scene->ctx, s.bvh_depth, P * std::max<uint64_t>(1, K), kPtBlock, false, cfg);
+  if (rc != MUONX_OK) return rc;
+  if ((rc = BeginSpill(scene->ctx, cfg, stream)) != MUONX_OK) return rc;
+  const int grid = cfg.grid;
   uint32_t *h_count = nullptr;
   if (p.max_depth == -1) MX_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_count), sizeof(uint32_t)));
-  const int stack_need = int(s.bvh_depth);
-  int rc = MUONX_OK;
   for (uint32_t p0 = 0; p0 < npix && rc == MUONX_OK; p0 += gen_pix) {
     const uint32_t np = std::min(gen_pix, npix - p0);
     for (uint32_t s0 = 0; s0 < uint32_t(p.spp); s0 += gen_samp) {
@@ -473,7 +487,6 @@ int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *ac
       g.nsamp = std::min<uint32_t>(gen_samp, uint32_t(p.spp) - s0);
       g.npaths = g.npix * g.nsamp;
       a.gen = g;
-      const int grid = int(std::min<uint64_t>(grid_cap, (g.npaths + kPtBlock - 1) / kPtBlock));
       MX_HIP(hipMemsetAsync(a.counts, 0, 4 * sizeof(uint32_t), stream));
       hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kPtBlock), 0, stream, a, g);
       // muon/integration.cc:29-33 at depth 0
@@ -481,14 +494,7 @@ int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *ac
         int cur = 0;
         const int max_bounces = p.max_depth == -1 ? 4096 : a.cutoff + 1;
         for (int depth = 0; depth < max_bounces; ++depth) {
-          if (stack_need <= 24) LaunchBounce<24>(a, cur, depth, grid, any_nee, mis, stream);
-          else if (stack_need <= 48) LaunchBounce<48>(a, cur, depth, grid, any_nee, mis, stream);
-          else if (stack_need <= 96) LaunchBounce<96>(a, cur, depth, grid, any_nee, mis, stream);
-          else {
-            SetError("BVH too deep for the traversal stack");
-            rc = MUONX_ERR_CAPACITY;
-            break;
-          }
+          DispatchTraverse(cfg, BounceLaunch{a, cfg, cur, depth, any_nee, mis, stream});
           MX_HIP(hipGetLastError());
           cur ^= 1;
           if (p.max_depth == -1 && (depth % 4 == 3)) {
@@ -499,11 +505,11 @@ int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *ac
           }
         }
       }
-      if (rc != MUONX_OK) break;
       hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(kPtBlock), 0, stream, a, g, accum);
       MX_HIP(hipGetLastError());
     }
   }
+  if (rc == MUONX_OK) rc = EndSpill(scene->ctx, cfg, stream);
   if (h_count) (void)hipHostFree(h_count);
   return rc;
 }

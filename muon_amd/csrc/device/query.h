@@ -4,13 +4,15 @@
 #include <hip/hip_runtime.h>
 
 #include "dev_scene.h"
+#include "runtime.h"
 
 namespace muonx {
 
 constexpr int kQueryBlock = 128;
 
-int LaunchQuery(const DevSceneView &s, bool any, bool exact, const float *ro, const float *rd, const float *tmax, uint32_t n,
-                int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters, hipStream_t stream);
+int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, const float *ro, const float *rd,
+                const float *tmax, uint32_t n, int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters,
+                hipStream_t stream);
 
 }  // namespace muonx
 

@@ -3,24 +3,25 @@
 #include "query.h"
 #include "runtime.h"
 #include "trace.h"
+#include "traverse_launch.h"
 
 namespace muonx {
 
-template <int STACK, bool ANY, bool COUNT, bool EXACT>
-__global__ void __launch_bounds__(kQueryBlock) k_query(DevSceneView s, const float *__restrict__ ro,
+template <int KIND, bool WW, bool ANY, bool COUNT, bool EXACT>
+__global__ void __launch_bounds__(kQueryBlock) k_query(DevSceneView s, StackArgs sa, const float *__restrict__ ro,
                                                        const float *__restrict__ rd, const float *__restrict__ tmax,
                                                        uint32_t n, int32_t *__restrict__ prim_out,
                                                        float *__restrict__ t_out, uint8_t *__restrict__ occ_out,
                                                        unsigned long long *__restrict__ counters) {
-  __shared__ uint2 stack[STACK * kQueryBlock];
-  const uint32_t i = blockIdx.x * kQueryBlock + threadIdx.x;
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kQueryBlock];
   TraceCounters cnt{0, 0};
-  if (i < n) {
+  for (uint32_t i = blockIdx.x * kQueryBlock + threadIdx.x; i < n; i += gridDim.x * kQueryBlock) {
     const V3 o{ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
     const V3 d{rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
     float best = tmax ? tmax[i] : __builtin_inff();
     const TraceRay r = make_ray(o, d);
-    const int32_t p = trace_bvh<STACK, ANY, COUNT, EXACT>(s, r, best, stack + threadIdx.x, kQueryBlock, cnt);
+    auto st = make_stack<KIND>(lds, kQueryBlock, sa);
+    const int32_t p = trace_bvh_s<ANY, COUNT, EXACT, WW>(s, r, best, st, cnt);
     if (ANY) {
       occ_out[i] = p >= 0 ? 1 : 0;
     } else {
@@ -42,48 +43,52 @@ __global__ void __launch_bounds__(kQueryBlock) k_query(DevSceneView s, const flo
   }
 }
 
+namespace {
 template <bool ANY, bool COUNT, bool EXACT>
-int LaunchQueryT(const DevSceneView &s, const float *ro, const float *rd, const float *tmax, uint32_t n,
-                 int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters, hipStream_t stream) {
-  if (n == 0) return MUONX_OK;
-  const dim3 grid((n + kQueryBlock - 1) / kQueryBlock);
-  const uint32_t need = s.bvh_depth;  // stack entries <= internal ancestors < depth
-  if (need <= 24) {
-    hipLaunchKernelGGL((k_query<24, ANY, COUNT, EXACT>), grid, dim3(kQueryBlock), 0, stream, s, ro, rd, tmax, n, prim, t,
-                       occ, counters);
-  } else if (need <= 48) {
-    hipLaunchKernelGGL((k_query<48, ANY, COUNT, EXACT>), grid, dim3(kQueryBlock), 0, stream, s, ro, rd, tmax, n, prim, t,
-                       occ, counters);
-  } else if (need <= 96) {
-    hipLaunchKernelGGL((k_query<96, ANY, COUNT, EXACT>), grid, dim3(kQueryBlock), 0, stream, s, ro, rd, tmax, n, prim, t,
-                       occ, counters);
-  } else {
-    SetError("BVH depth " + std::to_string(need) + " exceeds the traversal stack (96)");
-    return MUONX_ERR_CAPACITY;
+struct QueryLaunch {
+  const TraverseCfg &cfg;
+  const DevSceneView &s;
+  const float *ro, *rd, *tmax;
+  uint32_t n;
+  int32_t *prim;
+  float *t;
+  uint8_t *occ;
+  unsigned long long *counters;
+  hipStream_t stream;
+  template <int KIND, bool WW>
+  void run() const {
+    const StackArgs sa{cfg.spill, cfg.gcap};
+    hipLaunchKernelGGL((k_query<KIND, WW, ANY, COUNT, EXACT>), dim3(cfg.grid), dim3(kQueryBlock), 0, stream, s, sa,
+                       ro, rd, tmax, n, prim, t, occ, counters);
   }
-  MX_HIP(hipGetLastError());
-  return MUONX_OK;
-}
+};
+}  // namespace
 
-int LaunchQuery(const DevSceneView &s, bool any, bool exact, const float *ro, const float *rd, const float *tmax,
-                uint32_t n, int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters, hipStream_t stream) {
-#define MX_Q(A, C, E) return LaunchQueryT<A, C, E>(s, ro, rd, tmax, n, prim, t, occ, counters, stream)
+int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, const float *ro, const float *rd,
+                const float *tmax, uint32_t n, int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters,
+                hipStream_t stream) {
+  if (n == 0) return MUONX_OK;
+  TraverseCfg cfg;
+  int rc = PlanTraverse(ctx, s.bvh_depth, n, kQueryBlock, exact, cfg);
+  if (rc != MUONX_OK) return rc;
+  if ((rc = BeginSpill(ctx, cfg, stream)) != MUONX_OK) return rc;
+#define MX_Q(A, C, E) DispatchTraverse(cfg, QueryLaunch<A, C, E>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, stream})
   const bool cnt = counters != nullptr;
-  if (any) {
-    if (exact) {
-      if (cnt) MX_Q(true, true, true);
-      MX_Q(true, false, true);
-    }
-    if (cnt) MX_Q(true, true, false);
-    MX_Q(true, false, false);
-  }
   if (exact) {
-    if (cnt) MX_Q(false, true, true);
-    MX_Q(false, false, true);
+    // exact mode: full LDS stack, one node or leaf step per iteration
+    if (any) MX_Q(true, false, true);
+    else if (cnt) MX_Q(false, true, true);
+    else MX_Q(false, false, true);
+  } else if (any) {
+    if (cnt) MX_Q(true, true, false);
+    else MX_Q(true, false, false);
+  } else {
+    if (cnt) MX_Q(false, true, false);
+    else MX_Q(false, false, false);
   }
-  if (cnt) MX_Q(false, true, false);
-  MX_Q(false, false, false);
 #undef MX_Q
+  MX_HIP(hipGetLastError());
+  return EndSpill(ctx, cfg, stream);
 }
 
 }  // namespace muonx
@@ -103,8 +108,9 @@ extern "C" int muonx_intersect(muonx_scene *scene, const muonx_query *q, void *s
     SetError("muonx_intersect: unknown traversal mode");
     return MUONX_ERR_INVALID;
   }
+  std::lock_guard<std::mutex> lock(scene->ctx->mu);
   MX_HIP(hipSetDevice(scene->ctx->device));
-  return LaunchQuery(scene->view, q->any_hit != 0, q->mode == MUONX_TRAVERSAL_EXACT, q->ray_o, q->ray_d, q->tmax,
+  return LaunchQuery(scene->ctx, scene->view, q->any_hit != 0, q->mode == MUONX_TRAVERSAL_EXACT, q->ray_o, q->ray_d, q->tmax,
                      q->n, q->prim_id, q->t, q->occluded, q->counters, static_cast<hipStream_t>(stream));
 }
 

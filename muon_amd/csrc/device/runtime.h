@@ -64,6 +64,10 @@ struct muonx_ctx {
   std::mutex mu;
   muonx::RenderWorkspace *ws = nullptr;
   int num_cus = 0;
+  muonx_tuning tuning{8, 1, 16, 2048};  // best of the sweep on the level-7 icosphere
+  muonx::DevBuf spill;  // per-thread traversal stack spill region
+  // Orders kernels that share `spill` when they are issued on different streams.
+  hipEvent_t spill_done = nullptr;
 };
 
 struct muonx_scene {

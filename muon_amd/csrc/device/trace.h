@@ -169,86 +169,186 @@ __device__ __forceinline__ bool prim_hit(const DevSceneView &s, uint32_t i, cons
   return true;
 }
 
+// Per-lane traversal stacks. Entries are (child ref, entry distance).
+//
+// LdsStack: CAP entries in LDS at base[k * stride] (stride = block size, so
+// lane accesses are bank-conflict free). The caller sizes CAP >= BVH depth.
+template <int CAP>
+struct LdsStack {
+  uint2 *base;
+  int stride;
+  int sp;
+  __device__ __forceinline__ LdsStack(uint2 *b, int s) : base(b), stride(s), sp(0) {}
+  __device__ __forceinline__ void push(uint2 v) {
+    if (sp < CAP) {
+      base[sp * stride] = v;
+      ++sp;
+    }
+  }
+  __device__ __forceinline__ bool empty() const { return sp == 0; }
+  __device__ __forceinline__ uint2 pop() {
+    --sp;
+    return base[sp * stride];
+  }
+};
+
+// RingStack: the top S entries (S a power of two) in an LDS ring, older
+// entries spilled to a per-thread global region g[k * gstride], k <
+// gcap. Keeps LDS use small (high occupancy) while supporting deep trees.
+template <int S>
+struct RingStack {
+  uint2 *base;
+  int stride;
+  uint2 *g;
+  int gstride;
+  int gcap;
+  int sp;
+  __device__ __forceinline__ RingStack(uint2 *b, int s, uint2 *gb, int gs, int gc)
+      : base(b), stride(s), g(gb), gstride(gs), gcap(gc), sp(0) {}
+  __device__ __forceinline__ void push(uint2 v) {
+    const int slot = sp & (S - 1);
+    if (sp >= S) {
+      if (sp - S >= gcap) return;  // sized from the BVH depth: never taken
+      g[(sp - S) * gstride] = base[slot * stride];
+    }
+    base[slot * stride] = v;
+    ++sp;
+  }
+  __device__ __forceinline__ bool empty() const { return sp == 0; }
+  __device__ __forceinline__ uint2 pop() {
+    --sp;
+    const int slot = sp & (S - 1);
+    const uint2 v = base[slot * stride];
+    if (sp >= S) base[slot * stride] = g[(sp - S) * gstride];
+    return v;
+  }
+};
+
 // Traverses the BVH. ANY: stop at the first primitive with 0 < dist < tmax
 // (HasIntersection); otherwise closest hit (Intersect). EXACT selects the
 // reference's division slab test (identical node visits, used by the
 // deterministic integrators and the exact query mode); the default is the
-// conservative reciprocal test. Returns the leaf-order
-// primitive index or -1; `best` holds the distance on return.
-// The stack lives at stack[k * stride] for k = 0..STACK-1.
-template <int STACK, bool ANY, bool COUNT, bool EXACT = false>
-__device__ __forceinline__ int32_t trace_bvh(const DevSceneView &s, const TraceRay &r, float &best, uint2 *stack,
-                                             int stride, TraceCounters &cnt) {
+// conservative reciprocal test. WW ("while-while"): each lane walks internal
+// nodes until it holds a leaf, then the wavefront tests leaves together,
+// instead of interleaving node and leaf steps lane by lane. Returns the
+// leaf-order primitive index or -1; `best` holds the distance on return.
+// One internal-node step: tests both child boxes, pushes the far child when
+// both are hit, and moves `ref` to the next node; returns false when the
+// stack is exhausted.
+template <bool COUNT, bool EXACT, class Stack>
+__device__ __forceinline__ bool node_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
+                                          uint32_t &ref, TraceCounters &cnt) {
+  const float4 *np = s.nodes + 4 * size_t(ref);
+  const float4 n0 = np[0], n1 = np[1], n2 = np[2];
+  const int4 n3 = *reinterpret_cast<const int4 *>(np + 3);
+  if (COUNT) cnt.nodes++;
+  float lo0, lo1;
+  bool h0, h1;
+  if (EXACT) {
+    h0 = box_hit_exact(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
+    h1 = box_hit_exact(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+  } else {
+    h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
+    h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+  }
+  const uint32_t axis = (uint32_t(n3.z) >> 29) & 3u;
+  const bool swap = (r.dneg >> axis) & 1u;  // near child is children[1]
+  const uint32_t near_ref = swap ? uint32_t(n3.y) : uint32_t(n3.x);
+  const uint32_t far_ref = swap ? uint32_t(n3.x) : uint32_t(n3.y);
+  const bool hn = swap ? h1 : h0;
+  const bool hf = swap ? h0 : h1;
+  const float lof = swap ? lo0 : lo1;
+  if (hn) {
+    if (hf) stack.push(make_uint2(far_ref, __float_as_uint(lof)));
+    ref = near_ref;
+    return true;
+  }
+  if (hf) {
+    ref = far_ref;
+    return true;
+  }
+  return false;
+}
+
+// Pops the next entry whose box starts before `best`.
+template <class Stack>
+__device__ __forceinline__ bool pop_next(Stack &stack, float best, uint32_t &ref) {
+  while (!stack.empty()) {
+    const uint2 e = stack.pop();
+    if (__uint_as_float(e.y) < best) {
+      ref = e.x;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Tests the primitives of the leaf at `ref`; returns true if ANY found a hit.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool leaf_step(const DevSceneView &s, const TraceRay &r, float &best, uint32_t ref,
+                                          ObjCache &oc, int32_t &hit_prim, TraceCounters &cnt) {
+  uint32_t i = ref & ~kLeafBit;
+  while (true) {
+    float dist;
+    uint32_t flags;
+    if (COUNT) cnt.prims++;
+    if (prim_hit(s, i, r, oc, dist, flags) && dist > 0.0f && dist < best) {
+      best = dist;
+      hit_prim = int32_t(i);
+      if (ANY) return true;
+    }
+    if (flags & 1u) return false;
+    ++i;
+  }
+}
+
+// Traverses the BVH. ANY: stop at the first primitive with 0 < dist < tmax
+// (HasIntersection); otherwise closest hit (Intersect). EXACT selects the
+// reference's division slab test (identical node visits, used by the
+// deterministic integrators and the exact query mode); the default is the
+// conservative reciprocal test. WW ("while-while"): every lane walks
+// internal nodes until it holds a leaf, then the wavefront tests leaves
+// together; otherwise each iteration does one node or one leaf step per lane.
+// Returns the leaf-order primitive index or -1; `best` holds the distance.
+template <bool ANY, bool COUNT, bool EXACT, bool WW, class Stack>
+__device__ __forceinline__ int32_t trace_bvh_s(const DevSceneView &s, const TraceRay &r, float &best, Stack &stack,
+                                               TraceCounters &cnt) {
   int32_t hit_prim = -1;
   ObjCache oc;
   oc.tid = -1;
   uint32_t ref = 0;  // root: internal node 0
-  int sp = 0;
-  while (true) {
-    if (!(ref & kLeafBit)) {
-      const float4 *np = s.nodes + 4 * size_t(ref);
-      const float4 n0 = np[0], n1 = np[1], n2 = np[2];
-      const int4 n3 = *reinterpret_cast<const int4 *>(np + 3);
-      if (COUNT) cnt.nodes++;
-      float lo0, lo1;
-      bool h0, h1;
-      if (EXACT) {
-        h0 = box_hit_exact(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
-        h1 = box_hit_exact(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+  if (WW) {
+    while (true) {
+      bool alive = true;
+      while (!(ref & kLeafBit)) {
+        if (!node_step<COUNT, EXACT>(s, r, best, stack, ref, cnt) && !pop_next(stack, best, ref)) {
+          alive = false;
+          break;
+        }
+      }
+      if (!alive) break;
+      if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
+      if (!pop_next(stack, best, ref)) break;
+    }
+  } else {
+    while (true) {
+      if (!(ref & kLeafBit)) {
+        if (node_step<COUNT, EXACT>(s, r, best, stack, ref, cnt)) continue;
       } else {
-        h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
-        h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+        if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
       }
-      const uint32_t axis = (uint32_t(n3.z) >> 29) & 3u;
-      const bool swap = (r.dneg >> axis) & 1u;  // near child is children[1]
-      const uint32_t near_ref = swap ? uint32_t(n3.y) : uint32_t(n3.x);
-      const uint32_t far_ref = swap ? uint32_t(n3.x) : uint32_t(n3.y);
-      const bool hn = swap ? h1 : h0;
-      const bool hf = swap ? h0 : h1;
-      const float lof = swap ? lo0 : lo1;
-      if (hn) {
-        if (hf) {
-          if (sp < STACK) {
-            stack[sp * stride] = make_uint2(far_ref, __float_as_uint(lof));
-            ++sp;
-          }
-        }
-        ref = near_ref;
-        continue;
-      }
-      if (hf) {
-        ref = far_ref;
-        continue;
-      }
-    } else {
-      uint32_t i = ref & ~kLeafBit;
-      while (true) {
-        float dist;
-        uint32_t flags;
-        if (COUNT) cnt.prims++;
-        if (prim_hit(s, i, r, oc, dist, flags) && dist > 0.0f && dist < best) {
-          best = dist;
-          hit_prim = int32_t(i);
-          if (ANY) return hit_prim;
-        }
-        if (flags & 1u) break;
-        ++i;
-      }
+      if (!pop_next(stack, best, ref)) break;
     }
-    // pop, skipping entries whose box starts beyond the current best
-    bool found = false;
-    while (sp > 0) {
-      --sp;
-      const uint2 e = stack[sp * stride];
-      if (__uint_as_float(e.y) < best) {
-        ref = e.x;
-        found = true;
-        break;
-      }
-    }
-    if (!found) break;
   }
   return hit_prim;
+}
+
+// Fixed LDS stack at stack[k * stride], k < STACK (STACK >= BVH depth).
+template <int STACK, bool ANY, bool COUNT, bool EXACT = false>
+__device__ __forceinline__ int32_t trace_bvh(const DevSceneView &s, const TraceRay &r, float &best, uint2 *stack,
+                                             int stride, TraceCounters &cnt) {
+  LdsStack<STACK> st(stack, stride);
+  return trace_bvh_s<ANY, COUNT, EXACT, false>(s, r, best, st, cnt);
 }
 
 }  // namespace muonx

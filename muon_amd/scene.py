@@ -89,6 +89,16 @@ class Context:
             lib.muonx_ctx_destroy(self.h)
             self.h = None
 
+    def get_tuning(self):
+        t = _lib.Tuning()
+        check(lib.muonx_ctx_get_tuning(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.get_tuning()
+        cur.update(kw)
+        check(lib.muonx_ctx_set_tuning(self.h, C.byref(_lib.Tuning(**cur))))
+
 
 class DeviceScene:
     """A scene resident in HBM (muonx_scene)."""
