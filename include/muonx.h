@@ -186,6 +186,7 @@ int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);
  * (muon/acceleration.h:26-53) with a device-resident BVH. Immutable after
  * creation and shareable by concurrent renders. */
 int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *desc, muonx_scene **out);
+/* Waits for server jobs that still render the scene, then frees it. */
 void muonx_scene_destroy(muonx_scene *scene);
 int muonx_scene_get_info(const muonx_scene *scene, muonx_scene_info *out);
 
@@ -241,6 +242,68 @@ int muonx_render(muonx_scene *scene, const muonx_render_params *params, float *a
  * such as the muonx CLI). accum_rgb: host pointer, H*W*3 floats. */
 int muonx_render_host(muonx_scene *scene, const muonx_render_params *params, float *accum_rgb,
                       muonx_trace_stats *stats);
+
+/* ------------------------------------------------------------ batched and served rendering
+ * No counterpart in the reference, which renders one scene per process
+ * (muon/muon.cc:48-49). Path-tracer jobs of any resident scenes share one
+ * persistent wavefront engine: a fixed pool of path slots that every
+ * iteration advances by one bounce and refills with camera samples of
+ * admitted jobs, so jobs join and leave between bounces. Each job's film
+ * is accumulated in 64-bit fixed point (2^-24 units) and its random
+ * numbers are keyed by (seed, key, pixel, sample): the image of a job does
+ * not depend on what else is in flight. */
+#define MUONX_PENDING 1 /* muonx_server_wait: the job has not finished yet */
+
+typedef struct muonx_engine_config {
+  uint32_t path_slots;          /* 0: as many as tuning.path_budget_mb holds */
+  uint32_t max_jobs;            /* in-flight job slots; 0: 1024 */
+  int32_t shadow_per_vertex;    /* largest NEE shadow-ray count per vertex a job
+                                   may need; 0: 4 */
+  uint32_t iterations_per_poll; /* engine iterations between completion checks; 0: 2 */
+  uint64_t film_budget_bytes;   /* HBM for in-flight films, 24 B per pixel; 0: 1 GiB */
+  int32_t use_graph;            /* replay each iteration as a captured hipGraph */
+} muonx_engine_config;
+
+typedef struct muonx_job {
+  muonx_scene *scene;            /* resident scene (its context = the engine's) */
+  muonx_render_params params;    /* integrator must be MUONX_INTEGRATOR_PATHTRACER */
+  uint32_t key;                  /* random stream: equal (params, key) give equal images */
+  float *accum_rgb;              /* device, H*W*3 float sums over samples, as muonx_render */
+} muonx_job;
+
+typedef struct muonx_job_result {
+  int32_t status;                /* MUONX_OK or an error code */
+  uint64_t submit_ns, admit_ns, done_ns; /* steady clock: arrival, admission, film written */
+} muonx_job_result;
+
+/* Renders n jobs through one engine and returns when all films are
+ * written. results (n entries) and stats may be NULL. */
+int muonx_render_batch(muonx_ctx *ctx, const muonx_job *jobs, uint32_t n, const muonx_engine_config *cfg,
+                       muonx_job_result *results, muonx_trace_stats *stats);
+
+/* Request loop: a thread that owns an engine, admits submitted jobs in
+ * arrival order as film memory (film_budget_bytes) and job slots free up,
+ * and keeps the engine iterating while work is in flight. */
+int muonx_server_create(muonx_ctx *ctx, const muonx_engine_config *cfg, muonx_server **out);
+/* Validates and queues a job; returns a ticket. Jobs that can never fit
+ * (film above the budget, too many shadow rays per vertex) are rejected
+ * with MUONX_ERR_CAPACITY. */
+int muonx_server_submit(muonx_server *srv, const muonx_job *job, uint64_t *ticket);
+/* Waits up to timeout_ms (< 0: forever) for a ticket. Returns MUONX_OK and
+ * fills `result` (the ticket is then consumed), MUONX_PENDING on timeout. */
+int muonx_server_wait(muonx_server *srv, uint64_t ticket, int64_t timeout_ms, muonx_job_result *result);
+
+typedef struct muonx_server_stats {
+  uint64_t submitted, completed, pending, in_flight;
+  uint64_t iterations;
+  uint64_t primary_rays, secondary_rays, shadow_rays;
+  uint64_t film_bytes_in_use, film_budget_bytes;
+  uint32_t path_slots;
+  int32_t shadow_per_vertex;
+} muonx_server_stats;
+int muonx_server_get_stats(muonx_server *srv, muonx_server_stats *out);
+/* Finishes every submitted job, then stops the thread and frees the engine. */
+void muonx_server_destroy(muonx_server *srv);
 
 /* Film output, muon::Film::WriteOutput (muon/film.cc:21-43): /spp,
  * pow(v, 1/gamma), clamp [0,1], *255 truncated. Host arrays. */

@@ -64,6 +64,29 @@ class SceneInfo(C.Structure):
                 ("bvh_depth", C.c_uint32), ("num_lights", C.c_uint32), ("num_quad_lights", C.c_uint32)]
 
 
+class EngineConfig(C.Structure):
+    _fields_ = [("path_slots", C.c_uint32), ("max_jobs", C.c_uint32), ("shadow_per_vertex", C.c_int32),
+                ("iterations_per_poll", C.c_uint32), ("film_budget_bytes", C.c_uint64), ("use_graph", C.c_int32)]
+
+
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", RenderParams), ("key", C.c_uint32), ("accum_rgb", C.c_void_p)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("submit_ns", C.c_uint64), ("admit_ns", C.c_uint64),
+                ("done_ns", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("pending", C.c_uint64),
+                ("in_flight", C.c_uint64), ("iterations", C.c_uint64), ("primary_rays", C.c_uint64),
+                ("secondary_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("film_bytes_in_use", C.c_uint64),
+                ("film_budget_bytes", C.c_uint64), ("path_slots", C.c_uint32), ("shadow_per_vertex", C.c_int32)]
+
+
+MUONX_PENDING = 1
+
 P = C.c_void_p
 # (name, restype, argtypes): every entry point declared in include/muonx.h
 SIGNATURES = [
@@ -87,6 +110,13 @@ SIGNATURES = [
     ("muonx_intersect", C.c_int, [P, C.POINTER(Query), P]),
     ("muonx_render", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC), P]),
     ("muonx_render_host", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC)]),
+    ("muonx_render_batch", C.c_int, [P, C.POINTER(Job), C.c_uint32, C.POINTER(EngineConfig),
+                                     C.POINTER(JobResult), C.POINTER(TraceStatsC)]),
+    ("muonx_server_create", C.c_int, [P, C.POINTER(EngineConfig), C.POINTER(P)]),
+    ("muonx_server_submit", C.c_int, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    ("muonx_server_wait", C.c_int, [P, C.c_uint64, C.c_int64, C.POINTER(JobResult)]),
+    ("muonx_server_get_stats", C.c_int, [P, C.POINTER(ServerStats)]),
+    ("muonx_server_destroy", None, [P]),
     ("muonx_film_to_rgb8", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),
     ("muonx_write_png", C.c_int, [C.c_char_p, P, C.c_int32, C.c_int32]),
 ]

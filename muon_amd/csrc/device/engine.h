@@ -1,0 +1,173 @@
+// Persistent wavefront engine: the path tracer (PathTracer,
+// muon/integration.cc:205-630) over a fixed pool of path slots shared by any
+// number of render jobs of any scenes.
+//
+// Each iteration advances every in-flight path by one bounce:
+//   extend   closest hit of the ray queue
+//   shade    emission, NEE shadow rays, MIS BRDF ray, continuation (+ RR)
+//   shadow   any hit of the shadow slots
+//   mis      closest hit of the MIS rays, light hits add the weighted emission
+//   retire   adds the NEE / MIS terms; finished paths add their radiance to
+//            their job's film and return their slot to the free list
+//   plan     decides how many free slots to refill
+//   fill     generates camera samples of admitted jobs into freed slots
+// so new jobs join between any two bounces (continuous batching). Films are
+// accumulated in 64-bit fixed point (2^-24 units): integer addition is
+// associative, so a job's film does not depend on what else is in flight.
+// Random numbers are keyed by (job seed, job key, pixel, sample).
+#ifndef MUONX_DEVICE_ENGINE_H_
+#define MUONX_DEVICE_ENGINE_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../../include/muonx.h"
+#include "render.h"
+#include "runtime.h"
+#include "traverse_launch.h"
+
+namespace muonx {
+
+constexpr int kEngBlock = 128;
+constexpr int kMaxBounces = 1024;  // hard cap for max_depth = -1 without Russian roulette
+constexpr double kFilmScale = 16777216.0;  // fixed-point units per radiance unit
+
+struct JobDev {
+  CamParams cam;
+  uint32_t scene;  // scene table slot
+  uint32_t npix;
+  uint32_t seed, key;
+  int min_depth, cutoff, nee, is_mode, rr, light_samples, strata, per_section, K;
+  unsigned long long first_item;  // global index of the job's first sample
+  unsigned long long nitems;      // W * H * spp
+  unsigned long long *film;       // W * H * 3 fixed-point sums
+  unsigned long long *done;       // finished samples
+};
+
+// Device-side scalars of the engine (one struct so that the host can update
+// the admission window with a single copy).
+struct EngCounters {
+  uint32_t qcount[2];  // ray queues
+  uint32_t mis_count;
+  uint32_t free_count;
+  uint32_t plan_n, plan_free_base;
+  unsigned long long plan_cursor;
+  unsigned long long cursor;       // next work item to generate
+  unsigned long long total_items;  // admitted work (host-written)
+  unsigned long long seq_lo, seq_hi;  // admitted jobs (host-written)
+  unsigned long long rays[3];      // primary, secondary, shadow
+};
+
+struct EngArgs {
+  const DevSceneView *scenes;
+  const JobDev *jobs;
+  const unsigned long long *job_first;  // first_item by job sequence % max_jobs
+  uint32_t max_jobs;
+  uint32_t capacity;
+  int kmax;
+  EngCounters *c;
+  // path slots
+  uint32_t *slot_job, *slot_pixel, *slot_sample, *ctr, *free_list;
+  int32_t *depth;
+  uint8_t *alive;
+  float *rad, *thr, *m_fac, *m_add;
+  // queues (2 float4 per entry: (o.xyz, slot), (d.xyz, 0))
+  float4 *q_ray[2];
+  float4 *m_ray;
+  int32_t *hit_prim;
+  // shadow slots (slot * kmax + k)
+  float4 *s_org, *s_dir;
+  float *s_contrib;
+  uint8_t *s_vis, *s_valid;
+};
+
+struct JobSpec {
+  uint32_t scene_slot;
+  muonx_render_params params;
+  uint32_t key;
+  unsigned long long *film;  // npix * 3 fixed-point sums, zeroed by Admit
+};
+
+class Engine {
+ public:
+  // capacity: path slots; max_jobs: job slots; kmax: shadow rays per vertex.
+  // The traversal variant follows `tuning` (a copy is kept).
+  static int Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity, uint32_t max_jobs, int kmax,
+                    bool use_graph, Engine **out);
+  ~Engine();
+
+  // Registers a resident scene (by id); returns its slot. Entries of
+  // destroyed scenes are reused.
+  int AddScene(muonx_scene *scene, uint32_t *slot);
+  // Shadow rays one vertex of a job generates; admission requires <= kmax.
+  static int ShadowRaysPerVertex(const muonx_scene *scene, const muonx_render_params &p);
+  // Admits a job (film and done counter are zeroed on the engine stream).
+  int Admit(const JobSpec &job, uint64_t *seq);
+  // Advances the admission window past finished jobs (all of their samples
+  // generated and retired).
+  void Retire(uint64_t seq_lo) { seq_lo_ = seq_lo; }
+  // Launches one iteration (all kernels) on the engine stream.
+  int Iterate();
+  // Enqueues a copy of the per-job-slot done counters (index = seq %
+  // max_jobs) into host buffer `b` (0 or 1) and marks it with an event.
+  int SnapshotDone(int b);
+  // Waits for snapshot `b`; done_host(b) is then valid.
+  int WaitDone(int b);
+  const unsigned long long *done_host(int b) const { return h_done_[b]; }
+  // Ray counters (primary, secondary, shadow) taken with snapshot `b`.
+  const unsigned long long *rays_host(int b) const { return h_done_[b] + max_jobs_; }
+  // Converts a job's fixed-point film to float sums on the engine stream.
+  int ResolveFilm(const unsigned long long *film, uint32_t npix, float *out);
+  // Ray counters (primary, secondary, shadow) since the last reset; waits.
+  int RayCounts(unsigned long long out[3]);
+  int ResetRayCounts();
+
+  hipStream_t stream() const { return stream_; }
+  uint32_t capacity() const { return capacity_; }
+  uint32_t max_jobs() const { return max_jobs_; }
+  int kmax() const { return kmax_; }
+  uint64_t seq_lo() const { return seq_lo_; }
+  uint64_t seq_hi() const { return seq_hi_; }
+  uint64_t total_items() const { return total_items_; }
+  muonx_ctx *ctx() const { return ctx_; }
+  uint64_t device_bytes() const { return device_bytes_; }
+
+ private:
+  Engine() = default;
+  int LaunchIteration(int cur);
+  int SyncWindow();
+
+  muonx_ctx *ctx_ = nullptr;
+  muonx_tuning tuning_{};
+  int num_cus_ = 0;
+  hipStream_t stream_ = nullptr;
+  uint32_t capacity_ = 0, max_jobs_ = 0;
+  int kmax_ = 1;
+  bool use_graph_ = false;
+  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  uint32_t max_depth_ = 1;
+  uint64_t device_bytes_ = 0;
+  int cur_ = 0;
+  uint64_t seq_lo_ = 0, seq_hi_ = 0, total_items_ = 0;
+  uint64_t window_written_[3] = {~0ull, ~0ull, ~0ull};
+  TraverseCfg cfg_{};
+  EngArgs a_{};
+  DevBuf scenes_, jobs_, job_first_, counters_, slots_, queues_, shadow_, spill_, done_;
+  unsigned long long *h_done_[2] = {nullptr, nullptr};
+  hipEvent_t done_ev_[2] = {nullptr, nullptr};
+  std::vector<DevSceneView> host_scenes_;
+  std::vector<muonx_scene *> scene_ptrs_;
+  std::vector<uint64_t> scene_ids_;
+  // pinned host staging
+  JobDev *h_jobs_ = nullptr;
+  unsigned long long *h_first_ = nullptr;
+  unsigned long long *h_window_ = nullptr;  // [seq_lo, seq_hi, total_items] x 4 ring
+  int window_slot_ = 0;
+  hipEvent_t window_ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+}  // namespace muonx
+
+#endif

@@ -1,0 +1,787 @@
+// Persistent wavefront engine (see engine.h): kernels and host driver.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "engine.h"
+#include "hitrec.h"
+#include "shading.h"
+
+namespace muonx {
+
+
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
+  const uint64_t active = __ballot(1);
+  const uint64_t mask = __ballot(want);
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = uint32_t(__ffsll((unsigned long long)active) - 1);
+  uint32_t base = 0;
+  if (lane == leader && mask) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+  base = uint32_t(__shfl(int(base), int(leader)));
+  return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) {
+  unsigned long long a = n;
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
+  if (__lane_id() == 0 && a) atomicAdd(ctr, a);
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+  if (!(v > 0.0f)) return 0ull;  // zero, negative or NaN
+  const double x = double(v) * kFilmScale + 0.5;
+  return x < 4.0e18 ? (unsigned long long)x : 4000000000000000000ull;
+}
+
+// ---------------------------------------------------------------- extend
+template <int KIND, bool WW>
+__global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs sa, int cur) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
+  const uint32_t n = a.c->qcount[cur];
+  const float4 *q = a.q_ray[cur];
+  uint32_t prim_rays = 0, sec_rays = 0;
+  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < n) {
+      const float4 qo = q[2 * i], qd = q[2 * i + 1];
+      const uint32_t slot = __float_as_uint(qo.w);
+      const JobDev &job = a.jobs[a.slot_job[slot]];
+      const int depth = a.depth[slot];
+      int32_t hit = -1;
+      // Integrator::Trace returns 0 past the depth cutoff (muon/integration.cc:29-33)
+      if (depth <= job.cutoff && depth < kMaxBounces) {
+        const TraceRay r = make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z});
+        float best = __builtin_inff();
+        TraceCounters c;
+        auto st = make_stack<KIND>(lds, kEngBlock, sa);
+        hit = trace_bvh_s<false, false, false, WW>(a.scenes[job.scene], r, best, st, c);
+        if (depth == 0) ++prim_rays;
+        else ++sec_rays;
+      }
+      a.hit_prim[i] = hit;
+    }
+  }
+  wave_count(prim_rays, &a.c->rays[0]);
+  wave_count(sec_rays, &a.c->rays[1]);
+}
+
+// ---------------------------------------------------------------- shade
+__global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
+  const uint32_t n = a.c->qcount[cur];
+  uint32_t *next_count = &a.c->qcount[cur ^ 1];
+  const float4 *q = a.q_ray[cur];
+  float4 *qn = a.q_ray[cur ^ 1];
+  const int kmax = a.kmax;
+  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+    const uint32_t i = base + threadIdx.x;
+    bool cont = false, want_mis = false;
+    uint32_t slot = 0;
+    V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0};
+    if (i < n) {
+      const float4 qo = q[2 * i], qd = q[2 * i + 1];
+      slot = __float_as_uint(qo.w);
+      const JobDev &job = a.jobs[a.slot_job[slot]];
+      const DevSceneView &s = a.scenes[job.scene];
+      const int32_t prim = a.hit_prim[i];
+      const int depth = a.depth[slot];
+      for (int k = 0; k < job.K; ++k) {
+        a.s_vis[size_t(slot) * kmax + k] = 0;
+        a.s_valid[size_t(slot) * kmax + k] = 0;
+      }
+      a.m_add[3 * slot] = 0.0f;
+      a.m_add[3 * slot + 1] = 0.0f;
+      a.m_add[3 * slot + 2] = 0.0f;
+      if (prim >= 0) {
+        const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
+        const HitRecord h = hit_record(s, uint32_t(prim), ro, rd);
+        const Mat m = load_mat(s.materials + size_t(s.prim_mat[prim]) * kMatStride);
+        const uint32_t key = path_key(job.seed, job.key, a.slot_pixel[slot], a.slot_sample[slot]);
+        uint32_t ctr = a.ctr[slot];
+        V3 T{a.thr[3 * slot], a.thr[3 * slot + 1], a.thr[3 * slot + 2]};
+        V3 rad{a.rad[3 * slot], a.rad[3 * slot + 1], a.rad[3 * slot + 2]};
+        shift = h.pos + kEps * h.normal;
+        if (depth >= job.min_depth) {
+          // emission (muon/integration.cc:257-264)
+          if (!((job.nee != MUONX_NEE_OFF && depth > 0) || dot(h.normal, -rd) < 0.0f)) rad = rad + T * m.emission;
+          if (job.nee != MUONX_NEE_OFF) {
+            // ShadeDirectNEE (muon/integration.cc:340-466): one shadow ray per light sample
+            const bool mis = job.nee == MUONX_NEE_MIS;
+            int k = 0;
+            for (uint32_t l = 0; l < s.num_lights; ++l) {
+              const float *L = s.lights + size_t(l) * kLightStride;
+              const int kind = int(L[0]);
+              if (kind != MUONX_LIGHT_QUAD) {
+                V3 ldir, col{L[1], L[2], L[3]};
+                float ldist;
+                if (kind == MUONX_LIGHT_POINT) {
+                  const V3 to_light = V3{L[4], L[5], L[6]} - h.pos;
+                  ldir = normalize(to_light);
+                  ldist = length(to_light);
+                  const float att = L[7] + L[8] * ldist + L[9] * ldist * ldist;
+                  col = col / att;
+                } else {
+                  ldir = V3{L[4], L[5], L[6]};
+                  ldist = __builtin_inff();
+                }
+                V3 irr = col;
+                if (ldist < __builtin_inff()) irr = irr / (ldist * ldist);
+                const float cosi = gmax(dot(h.normal, ldir), 0.0f);
+                const V3 contrib = T * (irr * cosi * brdf_eval(m, ldir, rd, h.normal));
+                const size_t sl = size_t(slot) * kmax + k;
+                a.s_contrib[3 * sl] = contrib.x;
+                a.s_contrib[3 * sl + 1] = contrib.y;
+                a.s_contrib[3 * sl + 2] = contrib.z;
+                a.s_dir[sl] = make_float4(ldir.x, ldir.y, ldir.z, ldist);
+                a.s_org[sl] = make_float4(shift.x, shift.y, shift.z, 0.0f);
+                a.s_valid[sl] = (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) ? 1 : 0;
+                ++k;
+                continue;
+              }
+              const QuadLight ql = load_quad(L);
+              const V3 rn = normalize(cross(ql.e0, ql.e1));
+              const float scale = ql.area / float(job.light_samples);
+              for (int si = 0; si < job.strata; ++si) {
+                for (int sj = 0; sj < job.strata; ++sj) {
+                  for (int sk = 0; sk < job.per_section; ++sk) {
+                    const float u = rng_uniform(key, ctr);
+                    const float v = rng_uniform(key, ctr);
+                    V3 lp = (ql.corner + (float(si) + u) / float(job.strata) * ql.e0) +
+                            (float(sj) + v) / float(job.strata) * ql.e1;
+                    lp = lp - kEps * rn;
+                    const V3 to_light = lp - shift;
+                    const V3 ldir = normalize(to_light);
+                    const float ldist = length(to_light);
+                    const float cosi = gmax(dot(h.normal, ldir), 0.0f);
+                    const float cosl = gmax(dot(rn, ldir), 0.0f);
+                    const float g = cosi * cosl / (ldist * ldist);
+                    V3 c = g * brdf_eval(m, ldir, rd, h.normal);
+                    if (mis) {
+                      const float lpdf = nee_pdf(s, shift, ldir);
+                      const float bpdf = is_pdf(job.is_mode, m, ldir, rd, h.normal);
+                      c = c * power_heuristic(lpdf, bpdf);
+                    }
+                    const V3 contrib = T * (ql.color * (c * scale));
+                    const size_t sl = size_t(slot) * kmax + k;
+                    a.s_contrib[3 * sl] = contrib.x;
+                    a.s_contrib[3 * sl + 1] = contrib.y;
+                    a.s_contrib[3 * sl + 2] = contrib.z;
+                    a.s_dir[sl] = make_float4(ldir.x, ldir.y, ldir.z, ldist);
+                    a.s_org[sl] = make_float4(shift.x, shift.y, shift.z, 0.0f);
+                    // a zero contribution cannot change the image: its ray is skipped
+                    a.s_valid[sl] = (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) ? 1 : 0;
+                    ++k;
+                  }
+                }
+              }
+            }
+            if (mis) {
+              // ShadeDirectImportanceSampling (muon/integration.cc:468-501)
+              V3 dir, nT;
+              float bpdf;
+              if (sample_reflection(job.is_mode, m, rd, h.normal, T, key, ctr, dir, bpdf, nT)) {
+                const float lpdf = nee_pdf(s, shift, dir);
+                const V3 f = nT * power_heuristic(bpdf, lpdf);
+                a.m_fac[3 * slot] = f.x;
+                a.m_fac[3 * slot + 1] = f.y;
+                a.m_fac[3 * slot + 2] = f.z;
+                want_mis = true;
+                mis_dir = dir;
+              }
+            }
+          }
+        }
+        // ShadeIndirect (muon/integration.cc:294-328)
+        V3 dir, nT;
+        float pdf;
+        if (sample_reflection(job.is_mode, m, rd, h.normal, T, key, ctr, dir, pdf, nT)) {
+          bool alive = true;
+          if (job.rr) {
+            const float qc = gmin(gmax(gmax(nT.x, nT.y), nT.z), 1.0f);
+            if (qc < rng_uniform(key, ctr)) alive = false;
+            else nT = nT / qc;
+          }
+          // Trace() returns 0 past the depth cutoff (muon/integration.cc:29-33)
+          if (alive && depth + 1 <= job.cutoff && depth + 1 < kMaxBounces) {
+            cont = true;
+            next_dir = dir;
+            T = nT;
+          }
+        }
+        a.ctr[slot] = ctr;
+        a.thr[3 * slot] = T.x;
+        a.thr[3 * slot + 1] = T.y;
+        a.thr[3 * slot + 2] = T.z;
+        a.rad[3 * slot] = rad.x;
+        a.rad[3 * slot + 1] = rad.y;
+        a.rad[3 * slot + 2] = rad.z;
+      }
+      a.alive[slot] = cont ? 1 : 0;
+      if (cont) a.depth[slot] = depth + 1;
+    }
+    const uint32_t ni = wave_append(cont, next_count);
+    if (cont) {
+      qn[2 * ni] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(slot));
+      qn[2 * ni + 1] = make_float4(next_dir.x, next_dir.y, next_dir.z, 0.0f);
+    }
+    const uint32_t mi = wave_append(want_mis, &a.c->mis_count);
+    if (want_mis) {
+      a.m_ray[2 * mi] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(slot));
+      a.m_ray[2 * mi + 1] = make_float4(mis_dir.x, mis_dir.y, mis_dir.z, 0.0f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- shadow (any hit)
+template <int KIND, bool WW>
+__global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs sa, int cur) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
+  const uint32_t kmax = uint32_t(a.kmax);
+  const uint32_t n = a.c->qcount[cur] * kmax;
+  const float4 *q = a.q_ray[cur];
+  uint32_t traced = 0;
+  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < n) {
+      const uint32_t e = i / kmax, k = i % kmax;
+      const uint32_t slot = __float_as_uint(q[2 * e].w);
+      const JobDev &job = a.jobs[a.slot_job[slot]];
+      const size_t sl = size_t(slot) * kmax + k;
+      if (int(k) < job.K && a.hit_prim[e] >= 0 && a.s_valid[sl]) {
+        const float4 so = a.s_org[sl], sd = a.s_dir[sl];
+        const TraceRay r = make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z});
+        float best = sd.w;
+        TraceCounters c;
+        auto st = make_stack<KIND>(lds, kEngBlock, sa);
+        const int32_t hit = trace_bvh_s<true, false, false, WW>(a.scenes[job.scene], r, best, st, c);
+        a.s_vis[sl] = hit < 0 ? 1 : 0;
+        ++traced;
+      }
+    }
+  }
+  wave_count(traced, &a.c->rays[2]);
+}
+
+// ---------------------------------------------------------------- MIS extend
+template <int KIND, bool WW>
+__global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
+  const uint32_t n = a.c->mis_count;
+  uint32_t traced = 0;
+  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < n) {
+      const float4 mo = a.m_ray[2 * i], md = a.m_ray[2 * i + 1];
+      const uint32_t slot = __float_as_uint(mo.w);
+      const JobDev &job = a.jobs[a.slot_job[slot]];
+      const DevSceneView &s = a.scenes[job.scene];
+      const TraceRay r = make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z});
+      float best = __builtin_inff();
+      TraceCounters c;
+      auto st = make_stack<KIND>(lds, kEngBlock, sa);
+      const int32_t hit = trace_bvh_s<false, false, false, WW>(s, r, best, st, c);
+      ++traced;
+      if (hit >= 0) {
+        const int32_t light = s.prim_light[hit];
+        if (light >= 0) {
+          const float *L = s.lights + size_t(light) * kLightStride;
+          a.m_add[3 * slot] = a.m_fac[3 * slot] * L[1];
+          a.m_add[3 * slot + 1] = a.m_fac[3 * slot + 1] * L[2];
+          a.m_add[3 * slot + 2] = a.m_fac[3 * slot + 2] * L[3];
+        }
+      }
+    }
+  }
+  wave_count(traced, &a.c->rays[1]);
+}
+
+// ---------------------------------------------------------------- retire
+// Adds each shaded path's NEE and MIS terms in a fixed order; finished paths
+// add their radiance to the job's film and free their slot.
+__global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
+  const uint32_t n = a.c->qcount[cur];
+  const float4 *q = a.q_ray[cur];
+  const int kmax = a.kmax;
+  for (uint32_t i = blockIdx.x * kEngBlock + threadIdx.x; i < n; i += gridDim.x * kEngBlock) {
+    const uint32_t slot = __float_as_uint(q[2 * i].w);
+    const JobDev &job = a.jobs[a.slot_job[slot]];
+    V3 rad{a.rad[3 * slot], a.rad[3 * slot + 1], a.rad[3 * slot + 2]};
+    if (a.hit_prim[i] >= 0) {
+      V3 direct{0.0f, 0.0f, 0.0f};
+      for (int k = 0; k < job.K; ++k) {
+        const size_t sl = size_t(slot) * kmax + k;
+        if (a.s_vis[sl]) direct = direct + V3{a.s_contrib[3 * sl], a.s_contrib[3 * sl + 1], a.s_contrib[3 * sl + 2]};
+      }
+      direct = direct + V3{a.m_add[3 * slot], a.m_add[3 * slot + 1], a.m_add[3 * slot + 2]};
+      rad = rad + direct;
+      a.rad[3 * slot] = rad.x;
+      a.rad[3 * slot + 1] = rad.y;
+      a.rad[3 * slot + 2] = rad.z;
+    }
+    if (!a.alive[slot]) {
+      unsigned long long *px = job.film + 3 * size_t(a.slot_pixel[slot]);
+      const unsigned long long fx = to_fixed(rad.x), fy = to_fixed(rad.y), fz = to_fixed(rad.z);
+      if (fx) atomicAdd(px, fx);
+      if (fy) atomicAdd(px + 1, fy);
+      if (fz) atomicAdd(px + 2, fz);
+      atomicAdd(job.done, 1ull);
+      a.free_list[atomicAdd(&a.c->free_count, 1u)] = slot;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- plan / fill
+__global__ void k_eng_plan(EngArgs a, int cur) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  EngCounters *c = a.c;
+  c->qcount[cur] = 0;
+  c->mis_count = 0;
+  const unsigned long long avail = c->total_items > c->cursor ? c->total_items - c->cursor : 0ull;
+  const uint32_t nfill = uint32_t(avail < c->free_count ? avail : c->free_count);
+  c->plan_n = nfill;
+  c->plan_free_base = c->free_count - nfill;
+  c->plan_cursor = c->cursor;
+  c->free_count -= nfill;
+  c->cursor += nfill;
+}
+
+__global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
+  const EngCounters *c = a.c;
+  const uint32_t n = c->plan_n;
+  const uint32_t free_base = c->plan_free_base;
+  const unsigned long long cursor = c->plan_cursor;
+  const unsigned long long lo = c->seq_lo, hi = c->seq_hi;
+  uint32_t *next_count = &a.c->qcount[cur ^ 1];
+  float4 *qn = a.q_ray[cur ^ 1];
+  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t slot = 0;
+    V3 o{0, 0, 0}, d{0, 0, 0};
+    if (valid) {
+      slot = a.free_list[free_base + i];
+      const unsigned long long item = cursor + i;
+      // last admitted job whose first item is <= item
+      unsigned long long l = lo, h = hi;  // invariant: job_first[l] <= item < job_first[h]
+      while (h - l > 1) {
+        const unsigned long long m = l + (h - l) / 2;
+        if (a.job_first[m % a.max_jobs] <= item) l = m;
+        else h = m;
+      }
+      const uint32_t js = uint32_t(l % a.max_jobs);
+      const JobDev &job = a.jobs[js];
+      const unsigned long long local = item - job.first_item;
+      const uint32_t sample = uint32_t(local / job.npix);
+      const uint32_t pixel = uint32_t(local % job.npix);
+      a.slot_job[slot] = js;
+      a.slot_pixel[slot] = pixel;
+      a.slot_sample[slot] = sample;
+      a.depth[slot] = 0;
+      a.ctr[slot] = 0;
+      a.alive[slot] = 1;
+      a.thr[3 * slot] = 1.0f;
+      a.thr[3 * slot + 1] = 1.0f;
+      a.thr[3 * slot + 2] = 1.0f;
+      a.rad[3 * slot] = 0.0f;
+      a.rad[3 * slot + 1] = 0.0f;
+      a.rad[3 * slot + 2] = 0.0f;
+      const int px = int(pixel % uint32_t(job.cam.width)), py = int(pixel / uint32_t(job.cam.width));
+      float x, y;
+      sample_position(px, py, int(sample), path_key(job.seed, job.key, pixel, sample), x, y);
+      cast_ray(job.cam, x, y, o, d);
+    }
+    const uint32_t qi = wave_append(valid, next_count);
+    if (valid) {
+      qn[2 * qi] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
+      qn[2 * qi + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
+  }
+}
+
+__global__ void k_eng_init(EngArgs a) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.capacity; i += gridDim.x * blockDim.x)
+    a.free_list[i] = a.capacity - 1 - i;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.c->free_count = a.capacity;
+}
+
+__global__ void k_film_resolve(const unsigned long long *__restrict__ film, uint32_t n, float *__restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = float(double(film[i]) / kFilmScale);
+}
+
+// ---------------------------------------------------------------- host driver
+namespace {
+
+// Sub-allocates `count` T at address `p` (256-byte aligned) and advances
+// `p`; with a zero base it only measures.
+template <typename T>
+T *Carve(uintptr_t &p, size_t count) {
+  T *r = reinterpret_cast<T *>(p);
+  p += (count * sizeof(T) + 255) & ~size_t(255);
+  return r;
+}
+
+size_t CarveSlots(uintptr_t base, size_t P, EngArgs &a) {
+  uintptr_t p = base;
+  a.slot_job = Carve<uint32_t>(p, P);
+  a.slot_pixel = Carve<uint32_t>(p, P);
+  a.slot_sample = Carve<uint32_t>(p, P);
+  a.ctr = Carve<uint32_t>(p, P);
+  a.free_list = Carve<uint32_t>(p, P);
+  a.depth = Carve<int32_t>(p, P);
+  a.alive = Carve<uint8_t>(p, P);
+  a.rad = Carve<float>(p, 3 * P);
+  a.thr = Carve<float>(p, 3 * P);
+  a.m_fac = Carve<float>(p, 3 * P);
+  a.m_add = Carve<float>(p, 3 * P);
+  return size_t(p - base);
+}
+
+size_t CarveQueues(uintptr_t base, size_t P, EngArgs &a) {
+  uintptr_t p = base;
+  a.q_ray[0] = Carve<float4>(p, 2 * P);
+  a.q_ray[1] = Carve<float4>(p, 2 * P);
+  a.m_ray = Carve<float4>(p, 2 * P);
+  a.hit_prim = Carve<int32_t>(p, P);
+  return size_t(p - base);
+}
+
+size_t CarveShadow(uintptr_t base, size_t P, size_t K, EngArgs &a) {
+  uintptr_t p = base;
+  a.s_org = Carve<float4>(p, P * K);
+  a.s_dir = Carve<float4>(p, P * K);
+  a.s_contrib = Carve<float>(p, 3 * P * K);
+  a.s_vis = Carve<uint8_t>(p, P * K);
+  a.s_valid = Carve<uint8_t>(p, P * K);
+  return size_t(p - base);
+}
+
+struct IterLaunch {
+  const EngArgs &a;
+  const TraverseCfg &cfg;
+  int cur;
+  hipStream_t st;
+  template <int KIND, bool WW>
+  void run() const {
+    const StackArgs sa{cfg.spill, cfg.gcap};
+    const dim3 grid(cfg.grid), block(kEngBlock);
+    hipLaunchKernelGGL((k_eng_extend<KIND, WW>), grid, block, 0, st, a, sa, cur);
+    hipLaunchKernelGGL(k_eng_shade, grid, block, 0, st, a, cur);
+    hipLaunchKernelGGL((k_eng_shadow<KIND, WW>), grid, block, 0, st, a, sa, cur);
+    hipLaunchKernelGGL((k_eng_mis<KIND, WW>), grid, block, 0, st, a, sa);
+    hipLaunchKernelGGL(k_eng_retire, grid, block, 0, st, a, cur);
+    hipLaunchKernelGGL(k_eng_plan, dim3(1), dim3(64), 0, st, a, cur);
+    hipLaunchKernelGGL(k_eng_fill, grid, block, 0, st, a, cur);
+  }
+};
+
+}  // namespace
+
+int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity, uint32_t max_jobs, int kmax,
+                   bool use_graph, Engine **out) {
+  *out = nullptr;
+  if (capacity == 0 || max_jobs == 0 || kmax < 1) {
+    SetError("engine: capacity, max_jobs and kmax must be positive");
+    return MUONX_ERR_INVALID;
+  }
+  if (capacity > (1u << 30) || uint64_t(capacity) * uint64_t(kmax) > (uint64_t(1) << 31)) {
+    SetError("engine: path slots x shadow rays per vertex must stay below 2^31");
+    return MUONX_ERR_INVALID;
+  }
+  Engine *e = new Engine();
+  e->ctx_ = ctx;
+  e->tuning_ = tuning;
+  e->num_cus_ = ctx->num_cus;
+  e->capacity_ = capacity;
+  e->max_jobs_ = max_jobs;
+  e->kmax_ = kmax;
+  e->use_graph_ = use_graph;
+  auto fail = [&](int code) {
+    delete e;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&e->stream_, hipStreamNonBlocking) != hipSuccess) {
+    SetError("engine: stream creation failed");
+    return fail(MUONX_ERR_HIP);
+  }
+  // The buffers are carved twice: from a null base to size them, then for real.
+  EngArgs &a = e->a_;
+  size_t slot_bytes = 0, queue_bytes = 0, shadow_bytes = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const uintptr_t b0 = pass ? reinterpret_cast<uintptr_t>(e->slots_.ptr) : 0;
+    const uintptr_t b1 = pass ? reinterpret_cast<uintptr_t>(e->queues_.ptr) : 0;
+    const uintptr_t b2 = pass ? reinterpret_cast<uintptr_t>(e->shadow_.ptr) : 0;
+    slot_bytes = CarveSlots(b0, capacity, a);
+    queue_bytes = CarveQueues(b1, capacity, a);
+    shadow_bytes = CarveShadow(b2, capacity, size_t(kmax), a);
+    if (pass == 0 &&
+        (e->slots_.Alloc(slot_bytes) != hipSuccess || e->queues_.Alloc(queue_bytes) != hipSuccess ||
+         e->shadow_.Alloc(shadow_bytes) != hipSuccess || e->counters_.Alloc(sizeof(EngCounters)) != hipSuccess ||
+         e->jobs_.Alloc(sizeof(JobDev) * max_jobs) != hipSuccess ||
+         e->job_first_.Alloc(sizeof(unsigned long long) * max_jobs) != hipSuccess ||
+         e->done_.Alloc(sizeof(unsigned long long) * max_jobs) != hipSuccess)) {
+      SetError("engine: out of device memory for " + std::to_string(capacity) + " path slots");
+      return fail(MUONX_ERR_NOMEM);
+    }
+  }
+  e->device_bytes_ = slot_bytes + queue_bytes + shadow_bytes;
+  if (hipHostMalloc(reinterpret_cast<void **>(&e->h_jobs_), sizeof(JobDev) * max_jobs) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->h_first_), sizeof(unsigned long long) * max_jobs) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->h_window_), sizeof(unsigned long long) * 3 * 4) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->h_done_[0]), sizeof(unsigned long long) * (max_jobs + 3)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->h_done_[1]), sizeof(unsigned long long) * (max_jobs + 3)) != hipSuccess) {
+    SetError("engine: pinned host allocation failed");
+    return fail(MUONX_ERR_NOMEM);
+  }
+  for (auto &ev : e->window_ev_)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(MUONX_ERR_HIP);
+  for (auto &ev : e->done_ev_)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(MUONX_ERR_HIP);
+  a.c = e->counters_.as<EngCounters>();
+  a.jobs = e->jobs_.as<JobDev>();
+  a.job_first = e->job_first_.as<unsigned long long>();
+  a.max_jobs = max_jobs;
+  a.capacity = capacity;
+  a.kmax = kmax;
+  a.scenes = nullptr;
+  if (hipMemsetAsync(a.c, 0, sizeof(EngCounters), e->stream_) != hipSuccess ||
+      hipMemsetAsync(e->done_.ptr, 0, sizeof(unsigned long long) * max_jobs, e->stream_) != hipSuccess)
+    return fail(MUONX_ERR_HIP);
+  hipLaunchKernelGGL(k_eng_init, dim3(256), dim3(256), 0, e->stream_, a);
+  if (hipGetLastError() != hipSuccess) return fail(MUONX_ERR_HIP);
+  *out = e;
+  return MUONX_OK;
+}
+
+Engine::~Engine() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto &g : graph_)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto &ev : window_ev_)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : done_ev_)
+    if (ev) (void)hipEventDestroy(ev);
+  if (h_jobs_) (void)hipHostFree(h_jobs_);
+  if (h_first_) (void)hipHostFree(h_first_);
+  if (h_window_) (void)hipHostFree(h_window_);
+  for (auto *h : h_done_)
+    if (h) (void)hipHostFree(h);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Engine::AddScene(muonx_scene *scene, uint32_t *slot) {
+  // entries are keyed by scene id (ids are never reused)
+  for (size_t i = 0; i < scene_ids_.size(); ++i) {
+    if (scene_ids_[i] == scene->id) {
+      *slot = uint32_t(i);
+      return MUONX_OK;
+    }
+  }
+  // the scene table and spill region may be rewritten or reallocated below
+  MX_HIP(hipStreamSynchronize(stream_));
+  size_t at = scene_ids_.size();
+  for (size_t i = 0; i < scene_ids_.size(); ++i) {
+    if (!SceneLive(scene_ids_[i])) {  // a destroyed scene's entry: no job can refer to it
+      at = i;
+      break;
+    }
+  }
+  if (at == scene_ids_.size()) {
+    scene_ids_.push_back(scene->id);
+    host_scenes_.push_back(scene->view);
+    scene_ptrs_.push_back(scene);
+  } else {
+    scene_ids_[at] = scene->id;
+    host_scenes_[at] = scene->view;
+    scene_ptrs_[at] = scene;
+  }
+  MX_HIP(scenes_.Reserve(sizeof(DevSceneView) * std::max<size_t>(16, host_scenes_.capacity())));
+  MX_HIP(hipMemcpyAsync(scenes_.ptr, host_scenes_.data(), sizeof(DevSceneView) * host_scenes_.size(),
+                        hipMemcpyHostToDevice, stream_));
+  MX_HIP(hipStreamSynchronize(stream_));  // host_scenes_ may reallocate later
+  a_.scenes = scenes_.as<DevSceneView>();
+  *slot = uint32_t(at);
+  // the traversal plan covers the deepest resident BVH
+  const uint32_t depth = std::max(max_depth_, scene->view.bvh_depth);
+  if (depth != max_depth_ || !cfg_.grid) {
+    max_depth_ = depth;
+    TraverseCfg cfg;
+    const int rc =
+        PlanTraverseInto(tuning_, num_cus_, max_depth_, uint64_t(capacity_) * kmax_, kEngBlock, spill_, cfg, false);
+    if (rc != MUONX_OK) return rc;
+    cfg_ = cfg;
+    for (auto &g : graph_) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  } else {
+    for (auto &g : graph_) {  // the scene table pointer may have moved
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
+  return MUONX_OK;
+}
+
+namespace {
+// Quad-light sampling grid of ShadeDirectNEE (muon/integration.cc:390-400):
+// strata x strata cells with `per` samples each. glm::sqrt(int) is a double
+// square root; non-positive sample counts draw nothing.
+void LightGrid(const muonx_render_params &p, int *strata, int *per) {
+  if (p.light_samples <= 0) {
+    *strata = p.light_stratify ? 0 : 1;
+    *per = p.light_stratify ? 1 : 0;
+    return;
+  }
+  *strata = p.light_stratify ? int(std::sqrt(double(p.light_samples))) : 1;
+  *per = p.light_stratify ? 1 : p.light_samples;
+}
+}  // namespace
+
+int Engine::ShadowRaysPerVertex(const muonx_scene *scene, const muonx_render_params &p) {
+  if (p.nee == MUONX_NEE_OFF || scene->view.num_lights == 0) return 1;
+  int strata, per;
+  LightGrid(p, &strata, &per);
+  int64_t K = 0;
+  for (uint32_t l = 0; l < scene->view.num_lights; ++l) {
+    if (int(scene->host_lights[size_t(l) * kLightStride]) == MUONX_LIGHT_QUAD) K += int64_t(strata) * strata * per;
+    else K += 1;
+  }
+  return int(std::min<int64_t>(std::max<int64_t>(1, K), INT_MAX));
+}
+
+int Engine::Admit(const JobSpec &js, uint64_t *seq) {
+  if (seq_hi_ - seq_lo_ >= max_jobs_) {
+    SetError("engine: all job slots are in use");
+    return MUONX_ERR_CAPACITY;
+  }
+  const muonx_render_params &p = js.params;
+  const muonx_scene *scene = scene_ptrs_.at(js.scene_slot);
+  const int K = ShadowRaysPerVertex(scene, p);
+  if (K > kmax_) {
+    SetError("engine: job needs " + std::to_string(K) + " shadow rays per vertex, engine has " +
+             std::to_string(kmax_));
+    return MUONX_ERR_CAPACITY;
+  }
+  const uint32_t slot = uint32_t(seq_hi_ % max_jobs_);
+  JobDev &j = h_jobs_[slot];
+  j.cam = MakeCam(p);
+  j.scene = js.scene_slot;
+  j.npix = uint32_t(p.width) * uint32_t(p.height);
+  j.seed = p.seed;
+  j.key = js.key;
+  j.min_depth = p.min_depth;
+  j.cutoff = p.max_depth == -1 ? INT_MAX : (p.nee != MUONX_NEE_OFF ? p.max_depth - 1 : p.max_depth);
+  j.nee = p.nee;
+  j.is_mode = p.importance_sampling;
+  j.rr = p.russian_roulette;
+  j.light_samples = std::max(1, p.light_samples);
+  LightGrid(p, &j.strata, &j.per_section);
+  j.K = (p.nee == MUONX_NEE_OFF || scene->view.num_lights == 0) ? 0 : K;
+  j.first_item = total_items_;
+  j.nitems = uint64_t(j.npix) * uint64_t(p.spp);
+  j.film = js.film;
+  j.done = done_.as<unsigned long long>() + slot;
+  h_first_[slot] = j.first_item;
+  MX_HIP(hipMemsetAsync(js.film, 0, size_t(j.npix) * 3 * sizeof(unsigned long long), stream_));
+  MX_HIP(hipMemsetAsync(j.done, 0, sizeof(unsigned long long), stream_));
+  MX_HIP(hipMemcpyAsync(jobs_.as<JobDev>() + slot, &j, sizeof(JobDev), hipMemcpyHostToDevice, stream_));
+  MX_HIP(hipMemcpyAsync(job_first_.as<unsigned long long>() + slot, &h_first_[slot], sizeof(unsigned long long), hipMemcpyHostToDevice,
+                        stream_));
+  *seq = seq_hi_;
+  ++seq_hi_;
+  total_items_ += j.nitems;
+  return MUONX_OK;
+}
+
+int Engine::SyncWindow() {
+  if (window_written_[0] == seq_lo_ && window_written_[1] == seq_hi_ && window_written_[2] == total_items_)
+    return MUONX_OK;
+  const int k = window_slot_;
+  window_slot_ = (window_slot_ + 1) & 3;
+  MX_HIP(hipEventSynchronize(window_ev_[k]));  // the copy that last used this staging slot
+  unsigned long long *w = h_window_ + 3 * k;
+  w[0] = total_items_;
+  w[1] = seq_lo_;
+  w[2] = seq_hi_;
+  // total_items, seq_lo, seq_hi are consecutive in EngCounters
+  MX_HIP(hipMemcpyAsync(&a_.c->total_items, w, 3 * sizeof(unsigned long long), hipMemcpyHostToDevice, stream_));
+  MX_HIP(hipEventRecord(window_ev_[k], stream_));
+  window_written_[0] = seq_lo_;
+  window_written_[1] = seq_hi_;
+  window_written_[2] = total_items_;
+  return MUONX_OK;
+}
+
+int Engine::LaunchIteration(int cur) {
+  DispatchTraverse(cfg_, IterLaunch{a_, cfg_, cur, stream_});
+  MX_HIP(hipGetLastError());
+  return MUONX_OK;
+}
+
+int Engine::Iterate() {
+  if (!a_.scenes) {
+    SetError("engine: no scene registered");
+    return MUONX_ERR_STATE;
+  }
+  int rc = SyncWindow();
+  if (rc != MUONX_OK) return rc;
+  if (use_graph_) {
+    if (!graph_[cur_]) {
+      hipGraph_t g;
+      MX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      rc = LaunchIteration(cur_);
+      hipGraph_t captured = nullptr;
+      const hipError_t ec = hipStreamEndCapture(stream_, &captured);
+      if (rc != MUONX_OK) return rc;
+      if (ec != hipSuccess) {
+        SetError(std::string("engine: graph capture failed: ") + hipGetErrorString(ec));
+        return MUONX_ERR_HIP;
+      }
+      g = captured;
+      MX_HIP(hipGraphInstantiate(&graph_[cur_], g, nullptr, nullptr, 0));
+      MX_HIP(hipGraphDestroy(g));
+    }
+    MX_HIP(hipGraphLaunch(graph_[cur_], stream_));
+  } else {
+    rc = LaunchIteration(cur_);
+    if (rc != MUONX_OK) return rc;
+  }
+  cur_ ^= 1;
+  return MUONX_OK;
+}
+
+int Engine::SnapshotDone(int b) {
+  MX_HIP(hipMemcpyAsync(h_done_[b], done_.ptr, sizeof(unsigned long long) * max_jobs_, hipMemcpyDeviceToHost,
+                        stream_));
+  MX_HIP(hipMemcpyAsync(h_done_[b] + max_jobs_, &a_.c->rays[0], 3 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, stream_));
+  MX_HIP(hipEventRecord(done_ev_[b], stream_));
+  return MUONX_OK;
+}
+
+int Engine::WaitDone(int b) {
+  MX_HIP(hipEventSynchronize(done_ev_[b]));
+  return MUONX_OK;
+}
+
+int Engine::ResolveFilm(const unsigned long long *film, uint32_t npix, float *out) {
+  const uint32_t n = npix * 3;
+  hipLaunchKernelGGL(k_film_resolve, dim3(std::min<uint32_t>(4096, (n + 255) / 256)), dim3(256), 0, stream_, film, n,
+                     out);
+  MX_HIP(hipGetLastError());
+  return MUONX_OK;
+}
+
+int Engine::ResetRayCounts() {
+  MX_HIP(hipMemsetAsync(&a_.c->rays[0], 0, 3 * sizeof(unsigned long long), stream_));
+  return MUONX_OK;
+}
+
+int Engine::RayCounts(unsigned long long out[3]) {
+  MX_HIP(hipMemcpyAsync(out, &a_.c->rays[0], 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+  MX_HIP(hipStreamSynchronize(stream_));
+  return MUONX_OK;
+}
+
+}  // namespace muonx

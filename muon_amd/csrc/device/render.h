@@ -54,20 +54,27 @@ __device__ __forceinline__ void sample_position(int px, int py, int sidx, uint32
   if (y == float(py) + 1.0f) y = nextafterf(y, -__builtin_inff());
 }
 
+class Scheduler;
+
 // Per-context device workspace: grown on demand, reused across renders.
 struct RenderWorkspace {
   DevBuf counters;
-  DevBuf paths;     // path-state SoA
-  DevBuf queues;    // ray queues
-  DevBuf misc;
-  DevBuf film;      // staging film of muonx_render_host
-  size_t path_capacity = 0;
+  DevBuf film;  // staging film of muonx_render_host
+  // path-tracer engine of muonx_render / muonx_render_batch
+  Scheduler *sched = nullptr;
+  muonx_tuning sched_tuning{};
+  muonx_engine_config sched_want{};
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 RenderWorkspace *GetWorkspace(muonx_ctx *ctx);
+void DestroyContextScheduler(RenderWorkspace *ws);
 CamParams MakeCam(const muonx_render_params &p);
-int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *accum, RenderWorkspace *ws,
-                     hipStream_t stream);
+// One path-tracer job on the context's engine, ordered after earlier work on
+// `stream` (later work on `stream` waits for the film). rays: primary,
+// secondary, shadow.
+int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *accum, hipStream_t stream,
+                     unsigned long long rays[3]);
 
 }  // namespace muonx
 

@@ -12,7 +12,12 @@
 
 namespace muonx {
 
-void DestroyWorkspace(RenderWorkspace *ws) { delete ws; }
+void DestroyWorkspace(RenderWorkspace *ws) {
+  DestroyContextScheduler(ws);
+  if (ws->ev_in) (void)hipEventDestroy(ws->ev_in);
+  if (ws->ev_out) (void)hipEventDestroy(ws->ev_out);
+  delete ws;
+}
 
 RenderWorkspace *GetWorkspace(muonx_ctx *ctx) {
   if (!ctx->ws) ctx->ws = new RenderWorkspace();
@@ -137,6 +142,8 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
   MX_HIP(hipEventRecord(e0, stream));
   int rc;
   unsigned long long *ctr = ws->counters.as<unsigned long long>();
+  bool host_rays = false;
+  unsigned long long rays[3] = {0, 0, 0};
   switch (p.integrator) {
     case MUONX_INTEGRATOR_NORMALS:
     case MUONX_INTEGRATOR_ALBEDO:
@@ -144,7 +151,8 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
       rc = RenderSimple(scene, p, accum, ctr + kCtrPrimary, stream);
       break;
     case MUONX_INTEGRATOR_PATHTRACER:
-      rc = RenderPathTracer(scene, p, accum, ws, stream);
+      rc = RenderPathTracer(scene, p, accum, stream, rays);
+      host_rays = true;
       break;
     default:
       SetError("muonx_render: integrator " + std::to_string(p.integrator) + " is not on the GPU render path");
@@ -162,9 +170,9 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
     MX_HIP(hipStreamSynchronize(stream));
     float ms = 0.0f;
     MX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    stats->primary_rays = h[kCtrPrimary];
-    stats->secondary_rays = h[kCtrSecondary];
-    stats->shadow_rays = h[kCtrShadow];
+    stats->primary_rays = host_rays ? rays[0] : h[kCtrPrimary];
+    stats->secondary_rays = host_rays ? rays[1] : h[kCtrSecondary];
+    stats->shadow_rays = host_rays ? rays[2] : h[kCtrShadow];
     stats->render_ns = uint64_t(double(ms) * 1e6);
   }
   (void)hipEventDestroy(e0);

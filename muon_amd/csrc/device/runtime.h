@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -57,6 +58,12 @@ struct DevBuf {
 
 struct RenderWorkspace;
 
+// Scenes get ids that are never reused, so caches keyed by id cannot mistake
+// a new scene for a destroyed one at the same address.
+uint64_t RegisterScene();
+void UnregisterScene(uint64_t id);
+bool SceneLive(uint64_t id);
+
 }  // namespace muonx
 
 struct muonx_ctx {
@@ -72,6 +79,8 @@ struct muonx_ctx {
 
 struct muonx_scene {
   muonx_ctx *ctx = nullptr;
+  uint64_t id = 0;
+  std::atomic<int> inflight{0};  // server jobs that use the scene
   muonx::DevBuf nodes, prims, xforms, prim_id, prim_mat, prim_light, prim_vn, prim_use_vn, materials, lights;
   muonx::DevSceneView view{};
   uint32_t bvh_depth = 0;

@@ -1,7 +1,11 @@
 // Scene upload: converts the host descriptor (parse-order primitives + the
 // binary SAH BVH in pre-order) into the device layout of dev_scene.h.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <mutex>
+#include <thread>
+#include <unordered_set>
 #include <vector>
 
 #include "../common/vecmath.h"
@@ -37,6 +41,30 @@ int Upload(DevBuf &buf, const std::vector<T> &host, uint64_t &total) {
 }
 
 }  // namespace
+
+namespace {
+std::mutex g_scene_mu;
+std::unordered_set<uint64_t> g_live_scenes;
+uint64_t g_next_scene_id = 1;
+}  // namespace
+
+uint64_t RegisterScene() {
+  std::lock_guard<std::mutex> lock(g_scene_mu);
+  const uint64_t id = g_next_scene_id++;
+  g_live_scenes.insert(id);
+  return id;
+}
+
+void UnregisterScene(uint64_t id) {
+  std::lock_guard<std::mutex> lock(g_scene_mu);
+  g_live_scenes.erase(id);
+}
+
+bool SceneLive(uint64_t id) {
+  std::lock_guard<std::mutex> lock(g_scene_mu);
+  return g_live_scenes.count(id) != 0;
+}
+
 }  // namespace muonx
 
 using namespace muonx;
@@ -226,12 +254,16 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   s->device_bytes = total;
   s->host_lights = std::move(lights);
   s->host_materials = std::move(mats);
+  s->id = RegisterScene();
   *out = s;
   return MUONX_OK;
 }
 
 extern "C" void muonx_scene_destroy(muonx_scene *s) {
   if (!s) return;
+  // server jobs still in flight keep using the scene's device memory
+  while (s->inflight.load() > 0) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+  UnregisterScene(s->id);
   if (s->ctx) (void)hipSetDevice(s->ctx->device);
   delete s;
 }

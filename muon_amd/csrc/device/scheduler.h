@@ -1,0 +1,107 @@
+// Host-side scheduler over one Engine: admission of render jobs by film
+// memory and job slots, the iterate / snapshot loop, and completion
+// (film resolve into the job's output). Single-threaded; the server wraps it
+// in its own thread, muonx_render and muonx_render_batch drive it inline.
+#ifndef MUONX_DEVICE_SCHEDULER_H_
+#define MUONX_DEVICE_SCHEDULER_H_
+
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+
+namespace muonx {
+
+// First-fit range allocator over one device buffer (films of in-flight jobs).
+class FilmPool {
+ public:
+  static constexpr uint64_t kAlign = 256;
+  int Init(uint64_t bytes);
+  uint64_t capacity() const { return cap_; }
+  uint64_t in_use() const { return used_; }
+  bool empty() const { return used_ == 0; }
+  // Returns false when no free range is large enough.
+  bool Alloc(uint64_t bytes, uint64_t *offset);
+  void Free(uint64_t offset, uint64_t bytes);
+  unsigned long long *at(uint64_t offset) const {
+    return reinterpret_cast<unsigned long long *>(buf_.as<char>() + offset);
+  }
+  static uint64_t Round(uint64_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+ private:
+  DevBuf buf_;
+  uint64_t cap_ = 0, used_ = 0;
+  std::map<uint64_t, uint64_t> free_;  // offset -> bytes, coalesced
+};
+
+struct SchedJob {
+  uint64_t ticket = 0;
+  muonx_job spec{};
+  uint32_t scene_slot = 0;
+  uint64_t seq = 0;
+  uint64_t film_off = 0, film_bytes = 0;
+  uint64_t nitems = 0;
+  uint64_t admit_snap = 0;  // first done-snapshot taken after admission
+  uint64_t seen_done = 0;   // finished samples at the last snapshot
+  int state = 0;  // kPending, kActive, kResolving, kDone
+  muonx_job_result result{};
+  hipEvent_t ev = nullptr;  // resolve finished
+};
+
+class Scheduler {
+ public:
+  enum { kPending = 0, kActive = 1, kResolving = 2, kDone = 3 };
+
+  static int Create(muonx_ctx *ctx, const muonx_tuning &tuning, const muonx_engine_config &cfg, Scheduler **out);
+  ~Scheduler();
+
+  // Checks that the job can ever run on this scheduler (params, film size,
+  // shadow rays per vertex).
+  int Validate(const muonx_job &job) const;
+  // Queues a validated job; `submit_ns` is its arrival time.
+  int Submit(const muonx_job &job, uint64_t ticket, uint64_t submit_ns);
+  // One round: admits what fits, launches `iterations_per_poll` engine
+  // iterations, harvests the previous round's done counters and finished
+  // resolves. Jobs that completed are appended to `finished`.
+  int Step(std::vector<SchedJob> *finished);
+  bool Idle() const { return pending_.empty() && active_.empty(); }
+  size_t pending() const { return pending_.size(); }
+  size_t in_flight() const { return active_.size(); }
+  Engine *engine() const { return eng_; }
+  uint64_t iterations() const { return iterations_; }
+  const unsigned long long *rays() const { return rays_; }
+  uint64_t film_in_use() const { return films_.in_use(); }
+  uint64_t film_capacity() const { return films_.capacity(); }
+  const muonx_engine_config &config() const { return cfg_; }
+
+ private:
+  Scheduler() = default;
+  int AdmitPending();
+  int Harvest(int b, std::vector<SchedJob> *finished);
+  int FinishResolves(std::vector<SchedJob> *finished, bool wait_all);
+  void AdvanceWindow();
+
+  muonx_ctx *ctx_ = nullptr;
+  muonx_engine_config cfg_{};
+  Engine *eng_ = nullptr;
+  FilmPool films_;
+  std::deque<SchedJob> pending_;
+  std::map<uint64_t, SchedJob> active_;  // by engine sequence (admission order)
+  std::vector<hipEvent_t> free_events_;
+  int snap_ = 0;
+  bool snap_pending_[2] = {false, false};
+  uint64_t snap_id_[2] = {0, 0};
+  uint64_t snap_count_ = 0;
+  uint64_t iterations_ = 0;
+  uint64_t progress_iter_ = 0;  // iteration count when a sample last finished
+  unsigned long long rays_[3] = {0, 0, 0};
+};
+
+uint64_t NowNs();
+
+}  // namespace muonx
+
+#endif

@@ -1,0 +1,310 @@
+// Scheduler over the persistent engine (see scheduler.h).
+#include "scheduler.h"
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+
+namespace muonx {
+
+uint64_t NowNs() {
+  return uint64_t(
+      std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+          .count());
+}
+
+// ---------------------------------------------------------------- film pool
+int FilmPool::Init(uint64_t bytes) {
+  bytes = Round(std::max<uint64_t>(bytes, kAlign));
+  MX_HIP(buf_.Alloc(bytes));
+  cap_ = bytes;
+  used_ = 0;
+  free_.clear();
+  free_[0] = bytes;
+  return MUONX_OK;
+}
+
+bool FilmPool::Alloc(uint64_t bytes, uint64_t *offset) {
+  bytes = Round(bytes);
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < bytes) continue;
+    *offset = it->first;
+    const uint64_t rest = it->second - bytes, at = it->first + bytes;
+    free_.erase(it);
+    if (rest) free_[at] = rest;
+    used_ += bytes;
+    return true;
+  }
+  return false;
+}
+
+void FilmPool::Free(uint64_t offset, uint64_t bytes) {
+  bytes = Round(bytes);
+  used_ -= bytes;
+  auto it = free_.emplace(offset, bytes).first;
+  auto next = std::next(it);
+  if (next != free_.end() && it->first + it->second == next->first) {  // merge with the following range
+    it->second += next->second;
+    free_.erase(next);
+  }
+  if (it != free_.begin()) {  // merge with the preceding range
+    auto prev = std::prev(it);
+    if (prev->first + prev->second == it->first) {
+      prev->second += it->second;
+      free_.erase(it);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- scheduler
+namespace {
+
+uint64_t FilmBytes(const muonx_render_params &p) {
+  return FilmPool::Round(uint64_t(p.width) * uint64_t(p.height) * 3 * sizeof(unsigned long long));
+}
+
+// Device bytes of one path slot (slot state 73, queues 100, 46 per shadow
+// ray), as Engine::Create lays them out.
+uint64_t PathSlotBytes(int kmax) { return 73 + 100 + 46 * uint64_t(kmax); }
+
+}  // namespace
+
+int Scheduler::Create(muonx_ctx *ctx, const muonx_tuning &tuning, const muonx_engine_config &in, Scheduler **out) {
+  *out = nullptr;
+  muonx_engine_config cfg = in;
+  if (cfg.max_jobs == 0) cfg.max_jobs = 1024;
+  if (cfg.shadow_per_vertex <= 0) cfg.shadow_per_vertex = 4;
+  if (cfg.iterations_per_poll == 0) cfg.iterations_per_poll = 2;
+  if (cfg.film_budget_bytes == 0) cfg.film_budget_bytes = uint64_t(1) << 30;
+  if (cfg.path_slots == 0) {
+    const uint64_t budget = uint64_t(std::max(16, tuning.path_budget_mb)) << 20;
+    cfg.path_slots = uint32_t(std::min<uint64_t>(budget / PathSlotBytes(cfg.shadow_per_vertex), uint64_t(1) << 26));
+    cfg.path_slots = std::max<uint32_t>(cfg.path_slots, 1024);
+  }
+  Scheduler *s = new Scheduler();
+  s->ctx_ = ctx;
+  s->cfg_ = cfg;
+  int rc = Engine::Create(ctx, tuning, cfg.path_slots, cfg.max_jobs, cfg.shadow_per_vertex, cfg.use_graph != 0,
+                          &s->eng_);
+  if (rc == MUONX_OK) rc = s->films_.Init(cfg.film_budget_bytes);
+  if (rc != MUONX_OK) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return MUONX_OK;
+}
+
+Scheduler::~Scheduler() {
+  if (eng_) (void)hipStreamSynchronize(eng_->stream());
+  for (auto &kv : active_)
+    if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+  for (auto ev : free_events_) (void)hipEventDestroy(ev);
+  delete eng_;
+}
+
+int Scheduler::Validate(const muonx_job &job) const {
+  if (!job.scene || !job.accum_rgb) {
+    SetError("job: scene and accum_rgb must be set");
+    return MUONX_ERR_INVALID;
+  }
+  if (job.scene->ctx != ctx_) {
+    SetError("job: the scene belongs to another context");
+    return MUONX_ERR_INVALID;
+  }
+  const muonx_render_params &p = job.params;
+  if (p.width <= 0 || p.height <= 0 || p.spp <= 0) {
+    SetError("job: width, height and spp must be positive");
+    return MUONX_ERR_INVALID;
+  }
+  if (uint64_t(p.width) * uint64_t(p.height) > (uint64_t(1) << 28)) {
+    SetError("job: more than 2^28 pixels");
+    return MUONX_ERR_INVALID;
+  }
+  if (!p.has_camera) {
+    SetError("job: scene has no camera");
+    return MUONX_ERR_INVALID;
+  }
+  if (p.integrator != MUONX_INTEGRATOR_PATHTRACER) {
+    SetError("job: the engine runs the path tracer only (integrator " + std::to_string(p.integrator) + ")");
+    return MUONX_ERR_INVALID;
+  }
+  if (FilmBytes(p) > films_.capacity()) {
+    SetError("job: film of " + std::to_string(FilmBytes(p)) + " bytes exceeds the film budget of " +
+             std::to_string(films_.capacity()));
+    return MUONX_ERR_CAPACITY;
+  }
+  const int K = Engine::ShadowRaysPerVertex(job.scene, p);
+  if (K > eng_->kmax()) {
+    SetError("job: needs " + std::to_string(K) + " shadow rays per vertex, the engine holds " +
+             std::to_string(eng_->kmax()));
+    return MUONX_ERR_CAPACITY;
+  }
+  return MUONX_OK;
+}
+
+int Scheduler::Submit(const muonx_job &job, uint64_t ticket, uint64_t submit_ns) {
+  SchedJob j;
+  j.ticket = ticket;
+  j.spec = job;
+  j.film_bytes = FilmBytes(job.params);
+  j.nitems = uint64_t(job.params.width) * uint64_t(job.params.height) * uint64_t(job.params.spp);
+  j.state = kPending;
+  j.result.status = MUONX_PENDING;
+  j.result.submit_ns = submit_ns;
+  pending_.push_back(j);
+  return MUONX_OK;
+}
+
+int Scheduler::AdmitPending() {
+  while (!pending_.empty()) {
+    SchedJob &j = pending_.front();
+    if (eng_->seq_hi() - eng_->seq_lo() >= eng_->max_jobs()) break;
+    uint64_t off = 0;
+    if (!films_.Alloc(j.film_bytes, &off)) break;  // admission by film memory, in arrival order
+    uint32_t slot = 0;
+    int rc = eng_->AddScene(j.spec.scene, &slot);
+    if (rc == MUONX_OK) {
+      JobSpec js;
+      js.scene_slot = slot;
+      js.params = j.spec.params;
+      js.key = j.spec.key;
+      js.film = films_.at(off);
+      rc = eng_->Admit(js, &j.seq);
+    }
+    if (rc != MUONX_OK) {
+      films_.Free(off, j.film_bytes);
+      return rc;
+    }
+    j.film_off = off;
+    j.scene_slot = slot;
+    j.state = kActive;
+    j.admit_snap = snap_count_;
+    if (active_.empty()) progress_iter_ = iterations_;
+    j.result.admit_ns = NowNs();
+    active_.emplace(j.seq, j);
+    pending_.pop_front();
+  }
+  return MUONX_OK;
+}
+
+void Scheduler::AdvanceWindow() {
+  uint64_t lo = eng_->seq_hi();
+  for (const auto &kv : active_) {
+    if (kv.second.state == kActive) {
+      lo = kv.first;
+      break;
+    }
+  }
+  eng_->Retire(lo);
+}
+
+int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
+  (void)finished;
+  const unsigned long long *done = eng_->done_host(b);
+  bool any_active = false;
+  for (auto &kv : active_) {
+    SchedJob &j = kv.second;
+    if (j.state != kActive) continue;
+    any_active = true;
+    if (j.admit_snap > snap_id_[b]) continue;  // snapshot older than the job's admission
+    const uint64_t d = done[j.seq % eng_->max_jobs()];
+    if (d > j.seen_done) {
+      j.seen_done = d;
+      progress_iter_ = iterations_;
+    }
+    if (d < j.nitems) continue;
+    const uint32_t npix = uint32_t(j.spec.params.width) * uint32_t(j.spec.params.height);
+    int rc = eng_->ResolveFilm(films_.at(j.film_off), npix, j.spec.accum_rgb);
+    if (rc != MUONX_OK) return rc;
+    if (free_events_.empty()) {
+      hipEvent_t ev;
+      MX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      free_events_.push_back(ev);
+    }
+    j.ev = free_events_.back();
+    free_events_.pop_back();
+    MX_HIP(hipEventRecord(j.ev, eng_->stream()));
+    // later work on the engine stream (a new job's film clear) runs after the resolve
+    films_.Free(j.film_off, j.film_bytes);
+    j.state = kResolving;
+  }
+  // Every generated path retires within kMaxBounces + 2 iterations, so a
+  // long stretch without a finished sample means the engine lost paths.
+  if (!any_active) progress_iter_ = iterations_;
+  if (iterations_ - progress_iter_ > 2 * uint64_t(kMaxBounces) + 64) {
+    SetError("engine: no sample finished in " + std::to_string(iterations_ - progress_iter_) + " iterations");
+    return MUONX_ERR_STATE;
+  }
+  const unsigned long long *r = eng_->rays_host(b);
+  rays_[0] = r[0];
+  rays_[1] = r[1];
+  rays_[2] = r[2];
+  AdvanceWindow();
+  return MUONX_OK;
+}
+
+int Scheduler::FinishResolves(std::vector<SchedJob> *finished, bool wait_all) {
+  for (auto it = active_.begin(); it != active_.end();) {
+    SchedJob &j = it->second;
+    if (j.state == kResolving) {
+      const hipError_t q = wait_all ? hipEventSynchronize(j.ev) : hipEventQuery(j.ev);
+      if (q == hipSuccess) {
+        j.result.done_ns = NowNs();
+        j.result.status = MUONX_OK;
+        j.state = kDone;
+        free_events_.push_back(j.ev);
+        j.ev = nullptr;
+        finished->push_back(j);
+        it = active_.erase(it);
+        continue;
+      }
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: keep it out of later launch checks
+      } else {
+        SetError(std::string("engine: ") + hipGetErrorString(q));
+        return MUONX_ERR_HIP;
+      }
+    }
+    ++it;
+  }
+  return MUONX_OK;
+}
+
+int Scheduler::Step(std::vector<SchedJob> *finished) {
+  int rc = AdmitPending();
+  if (rc != MUONX_OK) return rc;
+  bool running = false;
+  for (const auto &kv : active_) running = running || kv.second.state == kActive;
+  if (running) {
+    for (uint32_t i = 0; i < cfg_.iterations_per_poll; ++i) {
+      if ((rc = eng_->Iterate()) != MUONX_OK) return rc;
+      ++iterations_;
+    }
+    const int b = snap_;
+    if ((rc = eng_->SnapshotDone(b)) != MUONX_OK) return rc;
+    snap_id_[b] = snap_count_++;
+    snap_pending_[b] = true;
+    snap_ ^= 1;
+    // the previous round's counters, read while this round runs
+    const int pb = b ^ 1;
+    if (snap_pending_[pb]) {
+      if ((rc = eng_->WaitDone(pb)) != MUONX_OK) return rc;
+      snap_pending_[pb] = false;
+      if ((rc = Harvest(pb, finished)) != MUONX_OK) return rc;
+    }
+  } else {
+    for (int k = 0; k < 2; ++k) {
+      const int b = snap_ ^ 1 ^ k;  // older snapshot first
+      if (!snap_pending_[b]) continue;
+      if ((rc = eng_->WaitDone(b)) != MUONX_OK) return rc;
+      snap_pending_[b] = false;
+      if ((rc = Harvest(b, finished)) != MUONX_OK) return rc;
+    }
+  }
+  bool still_running = false;
+  for (const auto &kv : active_) still_running = still_running || kv.second.state == kActive;
+  return FinishResolves(finished, !still_running && pending_.empty());
+}
+
+}  // namespace muonx

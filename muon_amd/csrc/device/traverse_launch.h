@@ -51,13 +51,12 @@ __device__ __forceinline__ typename StackOf<KIND>::T make_stack(uint2 *lds, int 
   }
 }
 
-// Resolves the variant for a scene and reserves the spill region. `want_exact`
-// restricts to full LDS stacks (the deterministic modes are not
-// performance-critical and keep the simplest form).
-inline int PlanTraverse(muonx_ctx *ctx, uint32_t bvh_depth, uint64_t work_items, int block, bool want_exact,
-                        TraverseCfg &cfg) {
-  const muonx_tuning &t = ctx->tuning;
-  const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+// Resolves the variant for a BVH depth and reserves the spill region in
+// `spill`. `want_exact` restricts to full LDS stacks (the deterministic
+// modes are not performance-critical and keep the simplest form).
+inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_depth, uint64_t work_items, int block,
+                            DevBuf &spill, TraverseCfg &cfg, bool want_exact = false) {
+  const int cus = num_cus > 0 ? num_cus : 256;
   const int bpc = std::max(1, t.blocks_per_cu);
   cfg.grid = int(std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * bpc, (work_items + block - 1) / block)));
   cfg.threads = cfg.grid * block;
@@ -65,32 +64,31 @@ inline int PlanTraverse(muonx_ctx *ctx, uint32_t bvh_depth, uint64_t work_items,
   cfg.spill = nullptr;
   cfg.gcap = 0;
   const int need = int(bvh_depth);  // stack entries never exceed depth - 1
-  if (!want_exact && (t.stack_entries == 8 || t.stack_entries == 16) && need > t.stack_entries) {
-    cfg.kind = t.stack_entries;
-    cfg.gcap = need - t.stack_entries;
-    MX_HIP(ctx->spill.Reserve(size_t(cfg.threads) * cfg.gcap * sizeof(uint2)));
-    cfg.spill = ctx->spill.as<uint2>();
+  const bool ring = !want_exact && (t.stack_entries == 8 || t.stack_entries == 16);
+  if (ring && need <= t.stack_entries) {
+    cfg.kind = need <= 8 ? 8 : 16;  // a ring that never spills
     return MUONX_OK;
   }
-  if (!want_exact && (t.stack_entries == 8 || t.stack_entries == 16) && need <= 16) {
-    cfg.kind = need <= 8 ? 8 : 16;  // a ring that never spills
-    cfg.gcap = 0;
+  if (ring || (!want_exact && need > 96)) {
+    cfg.kind = ring ? t.stack_entries : 16;
+    cfg.gcap = need - cfg.kind;
+    MX_HIP(spill.Reserve(size_t(cfg.threads) * cfg.gcap * sizeof(uint2)));
+    cfg.spill = spill.as<uint2>();
     return MUONX_OK;
   }
   if (need <= 24) cfg.kind = 24;
   else if (need <= 48) cfg.kind = 48;
   else if (need <= 96) cfg.kind = 96;
   else {
-    if (want_exact) {
-      SetError("BVH depth " + std::to_string(need) + " exceeds the exact-mode traversal stack (96)");
-      return MUONX_ERR_CAPACITY;
-    }
-    cfg.kind = 16;
-    cfg.gcap = need - 16;
-    MX_HIP(ctx->spill.Reserve(size_t(cfg.threads) * cfg.gcap * sizeof(uint2)));
-    cfg.spill = ctx->spill.as<uint2>();
+    SetError("BVH depth " + std::to_string(need) + " exceeds the exact-mode traversal stack (96)");
+    return MUONX_ERR_CAPACITY;
   }
   return MUONX_OK;
+}
+
+inline int PlanTraverse(muonx_ctx *ctx, uint32_t bvh_depth, uint64_t work_items, int block, bool want_exact,
+                        TraverseCfg &cfg) {
+  return PlanTraverseInto(ctx->tuning, ctx->num_cus, bvh_depth, work_items, block, ctx->spill, cfg, want_exact);
 }
 
 // Kernels that use the spill region wait for its previous user when issued

@@ -1,0 +1,209 @@
+"""Persistent engine: batched and served path tracing.
+
+A job's film depends only on (scene, params, key): fixed-point film sums
+and random numbers keyed by (seed, key, pixel, sample). So every job must
+come out bit-identical whether it runs alone (muonx_render), in a batch of
+uneven jobs (muonx_render_batch), through the request loop while other
+requests come and go (muonx_server_*), or through a pool of path slots far
+smaller than the job (slot regeneration)."""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from muon_amd import Context, DeviceScene, HostScene, MuonxError, scenes  # noqa: E402
+from muon_amd import _lib  # noqa: E402
+from muon_amd.renderer import render_device  # noqa: E402
+from muon_amd.serving import Server, engine_config, render_batch  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return Context(0)
+
+
+@pytest.fixture(scope="module")
+def resident(ctx):
+    """Three resident scenes of different kinds and sizes."""
+    out = {}
+    out["cornell"] = HostScene.from_text(scenes.cornell_text(64, 64, 4, 5))
+    out["ico"] = HostScene.from_text(scenes.icosphere_text(48, 40, 4, 8, level=3))
+    out["spheres"] = HostScene.from_text(scenes.sphere_field_text(40, 56, 4, 6, n=300))
+    return {k: (hs, DeviceScene(ctx, hs)) for k, hs in out.items()}
+
+
+def params_of(hs, **kw):
+    p = _lib.RenderParams.from_buffer_copy(hs.params)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def alone(ds, p):
+    acc, _ = render_device(ds, p)
+    torch.cuda.synchronize()
+    return acc.cpu().numpy()
+
+
+def job_list(resident):
+    """Uneven jobs: sizes, spp, depth, seeds and integrator options differ."""
+    jobs = []
+    specs = [
+        ("cornell", dict(width=64, height=64, spp=4, max_depth=5, seed=0)),
+        ("ico", dict(width=48, height=40, spp=3, max_depth=8, seed=5)),
+        ("spheres", dict(width=40, height=56, spp=2, max_depth=6, seed=9)),
+        ("cornell", dict(width=32, height=24, spp=7, max_depth=3, seed=11, nee=1)),
+        ("cornell", dict(width=50, height=30, spp=2, max_depth=4, seed=2, nee=0, russian_roulette=1)),
+        ("ico", dict(width=16, height=16, spp=9, max_depth=2, seed=1)),
+    ]
+    for name, kw in specs:
+        hs, ds = resident[name]
+        jobs.append((ds, params_of(hs, **kw), 0))
+    return jobs
+
+
+def test_batch_equals_alone(ctx, resident):
+    jobs = job_list(resident)
+    ref = [alone(ds, p) for ds, p, _ in jobs]
+    accs, results, st = render_batch(ctx, jobs)
+    torch.cuda.synchronize()
+    for i, (a, r) in enumerate(zip(accs, ref)):
+        got = a.cpu().numpy()
+        assert got.shape == r.shape
+        assert np.array_equal(got.view(np.uint32), r.view(np.uint32)), f"job {i}"
+        assert results[i].status == 0
+        assert results[i].submit_ns <= results[i].admit_ns <= results[i].done_ns
+    assert st.primary_rays >= sum(p.width * p.height * p.spp for _, p, _ in jobs) * 0.5
+
+
+def test_small_slot_pool_regenerates_paths(ctx, resident):
+    """1024 path slots for ~40k samples: paths are refilled many times."""
+    jobs = job_list(resident)[:3]
+    ref = [alone(ds, p) for ds, p, _ in jobs]
+    cfg = engine_config(path_slots=1024, max_jobs=4, shadow_per_vertex=2, iterations_per_poll=3)
+    accs, results, _ = render_batch(ctx, jobs, cfg)
+    torch.cuda.synchronize()
+    for a, r in zip(accs, ref):
+        assert np.array_equal(a.cpu().numpy().view(np.uint32), r.view(np.uint32))
+    # a job slot pool of 4 with 3 jobs, then a second batch on the same engine
+    accs2, _, _ = render_batch(ctx, jobs[::-1], cfg)
+    torch.cuda.synchronize()
+    for a, r in zip(accs2, ref[::-1]):
+        assert np.array_equal(a.cpu().numpy().view(np.uint32), r.view(np.uint32))
+
+
+def test_keys_change_the_noise_not_the_estimate(ctx, resident):
+    hs, ds = resident["cornell"]
+    p = params_of(hs, width=32, height=32, spp=8, max_depth=5, seed=0)
+    accs, _, _ = render_batch(ctx, [(ds, p, 0), (ds, p, 1), (ds, p, 0)])
+    a0, a1, a2 = (a.cpu().numpy().astype(np.float64) for a in accs)
+    assert np.array_equal(a0, a2)
+    assert not np.array_equal(a0, a1)
+    # independent estimates of one image: the mean per-pixel difference is
+    # zero up to its standard error
+    d = (a0 - a1).sum(axis=2).ravel()
+    assert abs(d.mean()) < 6.0 * d.std() / np.sqrt(d.size)
+
+
+def test_server_results_equal_alone_under_load(ctx, resident):
+    """Requests arrive while others run; the film budget admits only a few
+    at a time, so later requests wait for films to free up."""
+    jobs = job_list(resident) * 3
+    ref = [alone(ds, p) for ds, p, _ in jobs]
+    biggest = max(p.width * p.height for _, p, _ in jobs) * 24
+    cfg = engine_config(path_slots=8192, max_jobs=8, shadow_per_vertex=4, iterations_per_poll=2,
+                        film_budget_bytes=2 * biggest + 4096)
+    with Server(ctx, cfg) as srv:
+        tickets = []
+        for i, (ds, p, key) in enumerate(jobs):
+            tickets.append(srv.submit(ds, p, key))
+            if i % 4 == 3:
+                time.sleep(0.002)
+        outs = [srv.wait(t, timeout_ms=60000) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(jobs) and st["submitted"] == len(jobs)
+    assert st["film_bytes_in_use"] == 0
+    for i, (out, r) in enumerate(zip(outs, ref)):
+        assert out is not None, f"request {i} timed out"
+        acc, res = out
+        assert res.status == 0 and res.submit_ns <= res.admit_ns <= res.done_ns
+        assert np.array_equal(acc.cpu().numpy().view(np.uint32), r.view(np.uint32)), f"request {i}"
+
+
+def test_server_rejects_what_can_never_fit(ctx, resident):
+    hs, ds = resident["cornell"]
+    cfg = engine_config(path_slots=4096, max_jobs=4, shadow_per_vertex=1, film_budget_bytes=64 * 64 * 24)
+    with Server(ctx, cfg) as srv:
+        with pytest.raises(MuonxError) as e:
+            srv.submit(ds, params_of(hs, width=128, height=128, spp=1))
+        assert e.value.code == -6
+        with pytest.raises(MuonxError) as e:
+            srv.submit(ds, params_of(hs, width=32, height=32, spp=1, light_samples=4, light_stratify=1))
+        assert e.value.code == -6  # 4 shadow rays per vertex, the engine holds 1
+        with pytest.raises(MuonxError) as e:
+            srv.submit(ds, params_of(hs, width=32, height=32, spp=1, integrator=0))
+        assert e.value.code == -1
+        # a valid request still runs after the rejections
+        t = srv.submit(ds, params_of(hs, width=32, height=32, spp=2))
+        acc, res = srv.wait(t, timeout_ms=60000)
+        assert res.status == 0 and float(acc.sum()) > 0
+        with pytest.raises(MuonxError):
+            srv.wait(t, timeout_ms=0)  # already collected
+
+
+def test_server_wait_timeout_and_destroy_drains(ctx, resident):
+    hs, ds = resident["ico"]
+    p = params_of(hs, width=64, height=64, spp=16, max_depth=8)
+    ref = alone(ds, p)
+    srv = Server(ctx, engine_config(path_slots=2048, max_jobs=4))
+    t = srv.submit(ds, p, 0)
+    out = srv.wait(t, timeout_ms=0)
+    if out is None:  # still running: collect it properly
+        out = srv.wait(t, timeout_ms=60000)
+    assert np.array_equal(out[0].cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    # destroy with a request outstanding finishes it first
+    t2 = srv.submit(ds, p, 0)
+    acc2 = srv._outputs[t2]
+    srv.close()
+    torch.cuda.synchronize()
+    assert np.array_equal(acc2.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_new_scene_at_a_recycled_address_is_not_confused(resident):
+    """A context's engine caches the scenes it has seen. A scene created after
+    another was destroyed can get the same host address; it must still be
+    rendered from its own BVH (regression: the cache was keyed by address)."""
+    import gc
+    ctx2 = Context(0)
+    hs_a = HostScene.from_text(scenes.cornell_text(32, 32, 2, 4))
+    ds_a = DeviceScene(ctx2, hs_a)
+    alone(ds_a, params_of(hs_a))
+    addr_a = ds_a.h.value
+    del ds_a
+    gc.collect()
+    hs_b, _ = resident["ico"]
+    p = params_of(hs_b, width=32, height=32, spp=2, max_depth=5)
+    ds_b = DeviceScene(ctx2, hs_b)
+    got = alone(ds_b, p)
+    fresh = alone(DeviceScene(Context(0), hs_b), p)
+    assert np.array_equal(got.view(np.uint32), fresh.view(np.uint32)), ("recycled" if ds_b.h.value == addr_a else "")
+
+
+def test_scene_destroy_waits_for_server_jobs(ctx, resident):
+    hs, _ = resident["cornell"]
+    p = params_of(hs, width=64, height=64, spp=32, max_depth=5)
+    ds = DeviceScene(ctx, hs)
+    ref = alone(ds, p)
+    with Server(ctx, engine_config(path_slots=4096, max_jobs=4)) as srv:
+        t = srv.submit(ds, p, 0)
+        acc = srv._outputs[t]
+        _lib.lib.muonx_scene_destroy(ds.h)  # blocks until the job is done
+        ds.h = None
+        out = srv.wait(t, timeout_ms=60000)
+    assert out is not None and out[1].status == 0
+    assert np.array_equal(acc.cpu().numpy().view(np.uint32), ref.view(np.uint32))
