@@ -17,8 +17,21 @@ CPU baseline: the CPU restatement of the reference (same BVH, same
 per-primitive object-space tests) on a bounded sample of the same rays,
 timed on this host.
 
-Also reported (outside the timed region): one full path-traced render of the
-same scene (512x512, 64 spp, depth 8, BRDF IS + NEE + MIS).
+Also reported (outside the timed region):
+* render: one full path-traced render of the same scene (512x512, 64 spp,
+  depth 8, BRDF IS + NEE + MIS).
+* batched: 64 render jobs of uneven size (resolution uniform in
+  {128^2, 256^2, 512^2, 1024^2}, spp uniform in [4, 64], depth 8, seeded
+  camera poses)
+  over the resident scene pool (icospheres of 1.3k..1.3M tris, sphere
+  fields), rendered as one batch on the persistent engine and as 64
+  sequential plain muonx_render calls; images/s of both and whether every
+  image is bit-identical between the two.
+* serving: the request loop under a Poisson stream of requests (resolution
+  {256^2, 512^2, 1024^2}, spp 8..128, random scene and pose) at a fixed rate chosen
+  as a load fraction of the batched throughput; images/s, p50/p99 latency
+  (submit to film written) and relMSE of a few requests against a
+  high-spp render of the same request.
 """
 import argparse
 import json
@@ -37,6 +50,132 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def build_pool(ctx, log_fn, kinds=None):
+    from muon_amd import DeviceScene, HostScene, scenes
+    pool = []
+    for kind, size in (kinds or scenes.POOL):
+        t0 = time.perf_counter()
+        hs = HostScene.from_text(scenes.pool_scene_text(kind, size))
+        ds = DeviceScene(ctx, hs)
+        pool.append((kind, size, hs, ds, ds.info()))
+        log_fn(f"bench: pool scene {kind} {size}: {ds.info()['num_prims']} prims, {time.perf_counter() - t0:.1f} s")
+    return pool
+
+
+def make_job(pool, rng, res_choices, spp_lo, spp_hi, depth=8):
+    from muon_amd import _lib, scenes
+    kind, size, hs, ds, _ = pool[int(rng.integers(len(pool)))]
+    p = _lib.RenderParams.from_buffer_copy(hs.params)
+    r = int(rng.choice(res_choices))
+    scenes.random_pose(p, kind, rng, r, r)
+    p.spp = int(rng.integers(spp_lo, spp_hi + 1))
+    p.max_depth = depth
+    p.seed = int(rng.integers(1 << 31))
+    return ds, p
+
+
+def bench_batched(ctx, pool, njobs, seed, log_fn):
+    import numpy as np
+    import torch
+    from muon_amd.renderer import render_device
+    from muon_amd.serving import engine_config, render_batch
+    rng = np.random.default_rng(seed)
+    # key 0 for every job: the plain path renders with key 0 too, so both
+    # paths must produce the same images (the jobs differ by seed and pose)
+    jobs = [make_job(pool, rng, [128, 256, 512, 1024], 4, 64) + (0,) for _ in range(njobs)]
+    samples = sum(p.width * p.height * p.spp for _, p, _ in jobs)
+    film = sum(p.width * p.height * 24 + 256 for _, p, _ in jobs)
+    cfg = engine_config(max_jobs=max(64, 2 * njobs), film_budget_bytes=film + (1 << 20), iterations_per_poll=4)
+    log_fn(f"bench: batched {njobs} jobs, {samples / 1e6:.0f} M samples")
+    render_batch(ctx, jobs[:2], cfg)  # engine + graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    accs, results, st = render_batch(ctx, jobs, cfg)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    rays = st.primary_rays + st.secondary_rays + st.shadow_rays
+    log_fn(f"bench: batched {tb:.2f} s; sequential plain renders")
+    t0 = time.perf_counter()
+    seq = []
+    for ds, p, _ in jobs:
+        a, _ = render_device(ds, p)
+        seq.append(a)
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    same = sum(int(torch.equal(a, b)) for a, b in zip(accs, seq))
+    lat = [r.latency_s for r in results]
+    return {"jobs": njobs, "samples": samples, "batched_s": round(tb, 3), "sequential_s": round(ts, 3),
+            "images_per_s": round(njobs / tb, 3), "sequential_images_per_s": round(njobs / ts, 3),
+            "speedup": round(ts / tb, 3), "mrays_per_s": round(rays / tb / 1e6, 1),
+            "msamples_per_s": round(samples / tb / 1e6, 1), "identical_to_plain": f"{same}/{njobs}",
+            "job_latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+            "job_latency_p99_s": round(float(np.percentile(lat, 99)), 3)}, samples / tb
+
+
+def relmse(x, r):
+    import numpy as np
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def bench_serving(ctx, pool, nreq, capacity_samples_s, load, seed, log_fn):
+    import numpy as np
+    import torch
+    from muon_amd.serving import Server, engine_config, render_batch
+    rng = np.random.default_rng(seed + 1)
+    reqs = [make_job(pool, rng, [256, 512, 1024], 8, 128) for _ in range(nreq)]
+    mean_samples = float(np.mean([p.width * p.height * p.spp for _, p in reqs]))
+    rate = load * capacity_samples_s / mean_samples
+    gaps = rng.exponential(1.0 / rate, nreq)
+    cfg = engine_config(max_jobs=256, film_budget_bytes=1 << 30, iterations_per_poll=2)
+    log_fn(f"bench: serving {nreq} requests at {rate:.2f}/s (load {load})")
+    outs = {}
+    with Server(ctx, cfg) as srv:
+        # warm the engine (graph capture, scene table) outside the measurement
+        ds0, p0 = reqs[0]
+        srv.wait(srv.submit(ds0, p0, 0), timeout_ms=-1)
+        warm = srv.stats()
+        t_start = time.perf_counter()
+        t_next = t_start
+        tickets = []
+        for i, (ds, p) in enumerate(reqs):
+            t_next += gaps[i]
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(ds, p, i))
+        results = []
+        for i, t in enumerate(tickets):
+            acc, r = srv.wait(t, timeout_ms=-1)
+            results.append(r)
+            outs[i] = acc
+        stats = srv.stats()
+    t_end = max(r.done_ns for r in results)
+    t_first = min(r.submit_ns for r in results)
+    makespan = (t_end - t_first) * 1e-9
+    lat = np.array([r.latency_s for r in results])
+    # relMSE of the four smallest requests against 16x their spp (another key)
+    small = sorted(range(nreq), key=lambda i: reqs[i][1].width * reqs[i][1].height * reqs[i][1].spp)[:4]
+    ref_jobs = []
+    for i in small:
+        ds, p = reqs[i]
+        from muon_amd import _lib
+        q = _lib.RenderParams.from_buffer_copy(p)
+        q.spp = p.spp * 16
+        ref_jobs.append((ds, q, 1000 + i))
+    refs, _, _ = render_batch(ctx, ref_jobs)
+    rel = []
+    for i, ref in zip(small, refs):
+        p = reqs[i][1]
+        rel.append(round(relmse(outs[i].cpu().numpy() / p.spp, ref.cpu().numpy() / (p.spp * 16)), 5))
+    return {"requests": nreq, "rate_per_s": round(rate, 3), "load": load,
+            "images_per_s": round(nreq / makespan, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "mean_latency_s": round(float(lat.mean()), 3),
+            "mean_queue_s": round(float(np.mean([r.queue_s for r in results])), 3),
+            "mrays_per_s": round(sum(stats[k] - warm[k] for k in ("primary_rays", "secondary_rays", "shadow_rays"))
+                                 / makespan / 1e6, 1),
+            "relmse_vs_16x_spp": rel, "relmse_requests": [f"{reqs[i][1].width}^2x{reqs[i][1].spp}" for i in small]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +185,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving workloads")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -163,6 +306,16 @@ def main():
                   "secondary_rays": rst.secondary_rays, "shadow_rays": rst.shadow_rays,
                   "mean_radiance": float(accum.mean().item()) / hs.params.spp}
 
+    batched = serving = None
+    if not args.no_batch:
+        pool = build_pool(ctx, log)
+        batched, cap = bench_batched(ctx, pool, args.batch_jobs, 7, log)
+        log(f"bench: batched {batched}")
+        serving = bench_serving(ctx, pool, args.serve_requests, cap, args.serve_load, 7, log)
+        log(f"bench: serving {serving}")
+        pool_desc = [f"{k}:{i['num_prims']}" for k, _, _, _, i in pool]
+        batched["scenes"] = pool_desc
+
     out = {
         "metric": METRIC, "value": round(mrays, 1), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(mean_ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -171,7 +324,7 @@ def main():
                                "(327,680 tris) in the Cornell enclosure",
                    "rays_per_step": args.rays, "prims": info["num_prims"], "bvh_depth": info["bvh_depth"],
                    "hit_fraction": round(hits / args.rays, 4), "wall_s": round(wall, 3)},
-        "roofline": roofline, "cpu_baseline": cpu, "render": render,
+        "roofline": roofline, "cpu_baseline": cpu, "render": render, "batched": batched, "serving": serving,
     }
     if cpu:
         out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)

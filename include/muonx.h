@@ -172,12 +172,15 @@ void muonx_ctx_destroy(muonx_ctx *ctx);
  * older entries spilled to a per-thread HBM region (more waves per CU).
  * while_while: lanes walk internal nodes until each holds a leaf, then test
  * leaves together. blocks_per_cu: persistent grid size. path_budget_mb:
- * HBM for the path tracer's per-path state and queues. */
+ * HBM for the path tracer's per-path state and queues. bvh_width: node
+ * width of the fast traversal mode (the exact mode always walks the
+ * reference's binary tree). */
 typedef struct muonx_tuning {
   int32_t stack_entries;
   int32_t while_while;
   int32_t blocks_per_cu;
   int32_t path_budget_mb;
+  int32_t bvh_width;  /* fast mode: 2 = binary nodes, 4 = 4-wide nodes collapsed from the same tree */
 } muonx_tuning;
 int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t);
 int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);

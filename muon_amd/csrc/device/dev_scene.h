@@ -12,6 +12,16 @@
 // children, and the near child (reference order: children[dir[axis] < 0]
 // first, muon/acceleration.cc:77-82,134-138) is chosen from the axis.
 //
+// Wide BVH: the binary tree collapsed to 4-ary nodes (each binary internal
+// node absorbs the larger-area internal children until it has 4), 128 B per
+// node = 8 x float4:
+//   w0 = child min.x[0..3]  w1 = child max.x[0..3]
+//   w2 = child min.y[0..3]  w3 = child max.y[0..3]
+//   w4 = child min.z[0..3]  w5 = child max.z[0..3]
+//   w6 = child refs[0..3] (node index, or leaf bit | first primitive)
+//   w7 = (child count, 0, 0, 0)
+// The same leaves (primitive ranges) as the binary tree.
+//
 // Primitives (leaf order), 48 B = 3 x float4:
 //   tri:    (v0.xyz, tid)  (v1.xyz, 0)  (v2.xyz, 0)     object space
 //   sphere: (c.xyz, tid | kSphereBit)  (r, 0, 0, 0)  (0)
@@ -37,6 +47,7 @@ constexpr int kLightStride = 16;
 
 struct DevSceneView {
   const float4 *nodes;      // 4 per internal node
+  const float4 *wnodes;     // 8 per wide node
   const float4 *prims;      // 3 per primitive, leaf order
   const float *xforms;      // 48 per transform
   const int32_t *prim_id;   // leaf order -> parse order
@@ -51,6 +62,8 @@ struct DevSceneView {
   uint32_t num_lights;
   uint32_t num_quad_lights;
   uint32_t bvh_depth;
+  uint32_t num_wnodes;
+  uint32_t wstack;          // bound on stack entries of a wide traversal
 };
 
 }  // namespace muonx

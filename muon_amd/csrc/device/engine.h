@@ -1,20 +1,23 @@
 // Persistent wavefront engine: the path tracer (PathTracer,
-// muon/integration.cc:205-630) over a fixed pool of path slots shared by any
-// number of render jobs of any scenes.
+// muon/integration.cc:205-630) over a fixed-capacity queue of paths shared
+// by any number of render jobs of any scenes.
 //
-// Each iteration advances every in-flight path by one bounce:
-//   extend   closest hit of the ray queue
-//   shade    emission, NEE shadow rays, MIS BRDF ray, continuation (+ RR)
-//   shadow   any hit of the shadow slots
+// Each iteration advances every queued path by one bounce:
+//   extend   closest hit of the queue's rays
+//   shade    emission, NEE shadow rays, MIS BRDF ray; a continuing path is
+//            appended to the next queue together with its state
+//   shadow   any hit of the compacted shadow rays
 //   mis      closest hit of the MIS rays, light hits add the weighted emission
-//   retire   adds the NEE / MIS terms; finished paths add their radiance to
-//            their job's film and return their slot to the free list
-//   plan     decides how many free slots to refill
-//   fill     generates camera samples of admitted jobs into freed slots
-// so new jobs join between any two bounces (continuous batching). Films are
-// accumulated in 64-bit fixed point (2^-24 units): integer addition is
-// associative, so a job's film does not depend on what else is in flight.
-// Random numbers are keyed by (job seed, job key, pixel, sample).
+//   retire   adds the NEE / MIS terms; continuing paths carry their radiance
+//            into the next queue, finished ones add it to their job's film
+//   plan     decides how many new samples fit behind the continuing paths
+//   fill     appends camera samples of admitted jobs, in work order
+// so new jobs join between any two bounces (continuous batching). Path state
+// moves with the queue (compacted every bounce), so every per-path access is
+// coalesced. Films are accumulated in 64-bit fixed point (2^-24 units):
+// integer addition is associative, so a job's film does not depend on what
+// else is in flight. Random numbers are keyed by (job seed, job key, pixel,
+// sample).
 #ifndef MUONX_DEVICE_ENGINE_H_
 #define MUONX_DEVICE_ENGINE_H_
 
@@ -49,15 +52,22 @@ struct JobDev {
 // Device-side scalars of the engine (one struct so that the host can update
 // the admission window with a single copy).
 struct EngCounters {
-  uint32_t qcount[2];  // ray queues
+  uint32_t qcount[2];  // path queues
   uint32_t mis_count;
-  uint32_t free_count;
-  uint32_t plan_n, plan_free_base;
+  uint32_t shadow_count;
+  uint32_t plan_n, plan_base;
   unsigned long long plan_cursor;
-  unsigned long long cursor;       // next work item to generate
-  unsigned long long total_items;  // admitted work (host-written)
+  unsigned long long cursor;          // next work item to generate
+  unsigned long long total_items;     // admitted work (host-written)
   unsigned long long seq_lo, seq_hi;  // admitted jobs (host-written)
-  unsigned long long rays[3];      // primary, secondary, shadow
+  unsigned long long rays[3];         // primary, secondary, shadow
+};
+
+// Per-path state, struct of arrays over queue positions.
+struct PathState {
+  uint32_t *job, *pixel, *sample, *ctr;
+  int32_t *depth;
+  float *rad, *thr;  // 3 per path
 };
 
 struct EngArgs {
@@ -68,19 +78,18 @@ struct EngArgs {
   uint32_t capacity;
   int kmax;
   EngCounters *c;
-  // path slots
-  uint32_t *slot_job, *slot_pixel, *slot_sample, *ctr, *free_list;
-  int32_t *depth;
-  uint8_t *alive;
-  float *rad, *thr, *m_fac, *m_add;
-  // queues (2 float4 per entry: (o.xyz, slot), (d.xyz, 0))
-  float4 *q_ray[2];
-  float4 *m_ray;
-  int32_t *hit_prim;
-  // shadow slots (slot * kmax + k)
-  float4 *s_org, *s_dir;
+  PathState st[2];    // state of queue 0 / 1
+  float4 *q_ray[2];   // 2 per path: (o.xyz, 0), (d.xyz, 0)
+  int32_t *hit_prim;  // closest hit of path i of the current queue
+  uint32_t *nidx;     // its position in the next queue, or ~0 when it ends
+  float4 *m_ray;      // MIS rays: (o.xyz, path index), (d.xyz, 0)
+  float *m_fac, *m_add;
+  float4 *shift;      // origin of path i's shadow and MIS rays
+  // shadow rays, index k * capacity + i, and the queue of those to trace
+  float4 *s_dir;      // (direction, tmax)
   float *s_contrib;
   uint8_t *s_vis, *s_valid;
+  uint32_t *s_queue;
 };
 
 struct JobSpec {
@@ -92,7 +101,7 @@ struct JobSpec {
 
 class Engine {
  public:
-  // capacity: path slots; max_jobs: job slots; kmax: shadow rays per vertex.
+  // capacity: paths in flight; max_jobs: job slots; kmax: shadow rays per vertex.
   // The traversal variant follows `tuning` (a copy is kept).
   static int Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity, uint32_t max_jobs, int kmax,
                     bool use_graph, Engine **out);
@@ -147,14 +156,14 @@ class Engine {
   int kmax_ = 1;
   bool use_graph_ = false;
   hipGraphExec_t graph_[2] = {nullptr, nullptr};
-  uint32_t max_depth_ = 1;
+  uint32_t max_depth_ = 1, max_wstack_ = 1;
   uint64_t device_bytes_ = 0;
   int cur_ = 0;
   uint64_t seq_lo_ = 0, seq_hi_ = 0, total_items_ = 0;
   uint64_t window_written_[3] = {~0ull, ~0ull, ~0ull};
   TraverseCfg cfg_{};
   EngArgs a_{};
-  DevBuf scenes_, jobs_, job_first_, counters_, slots_, queues_, shadow_, spill_, done_;
+  DevBuf scenes_, jobs_, job_first_, counters_, paths_, shadow_, spill_, done_;
   unsigned long long *h_done_[2] = {nullptr, nullptr};
   hipEvent_t done_ev_[2] = {nullptr, nullptr};
   std::vector<DevSceneView> host_scenes_;

@@ -22,6 +22,23 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
   return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// Reserves `cnt` consecutive entries per lane with one atomic per wavefront
+// (exclusive prefix sum over the lanes); all 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t cnt, uint32_t *counter) {
+  const int lane = int(__lane_id());
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = uint32_t(__shfl_up(int(incl), off));
+    if (lane >= off) incl += v;
+  }
+  const uint32_t total = uint32_t(__shfl(int(incl), 63));
+  uint32_t base = 0;
+  if (lane == 63 && total) base = atomicAdd(counter, total);
+  base = uint32_t(__shfl(int(base), 63));
+  return base + incl - cnt;
+}
+
 __device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) {
   unsigned long long a = n;
   for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
@@ -35,27 +52,30 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 }
 
 // ---------------------------------------------------------------- extend
-template <int KIND, bool WW>
+template <int KIND, bool WW, int WIDTH>
 __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t n = a.c->qcount[cur];
   const float4 *q = a.q_ray[cur];
+  const PathState &ps = a.st[cur];
   uint32_t prim_rays = 0, sec_rays = 0;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t i = base + threadIdx.x;
     if (i < n) {
       const float4 qo = q[2 * i], qd = q[2 * i + 1];
-      const uint32_t slot = __float_as_uint(qo.w);
-      const JobDev &job = a.jobs[a.slot_job[slot]];
-      const int depth = a.depth[slot];
+      const uint32_t js = ps.job[i];
+      const int depth = ps.depth[i];
       int32_t hit = -1;
       // Integrator::Trace returns 0 past the depth cutoff (muon/integration.cc:29-33)
-      if (depth <= job.cutoff && depth < kMaxBounces) {
+      if (depth <= a.jobs[js].cutoff && depth < kMaxBounces) {
+        // a register copy: the traversal's global spill stores must not force
+        // re-reads of the scene's pointers
+        const DevSceneView sv = a.scenes[a.jobs[js].scene];
         const TraceRay r = make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z});
         float best = __builtin_inff();
         TraceCounters c;
         auto st = make_stack<KIND>(lds, kEngBlock, sa);
-        hit = trace_bvh_s<false, false, false, WW>(a.scenes[job.scene], r, best, st, c);
+        hit = trace_fast<WIDTH, false, false, WW>(sv, r, best, st, c);
         if (depth == 0) ++prim_rays;
         else ++sec_rays;
       }
@@ -69,38 +89,45 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
 // ---------------------------------------------------------------- shade
 __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
   const uint32_t n = a.c->qcount[cur];
-  uint32_t *next_count = &a.c->qcount[cur ^ 1];
   const float4 *q = a.q_ray[cur];
   float4 *qn = a.q_ray[cur ^ 1];
-  const int kmax = a.kmax;
+  const PathState &ps = a.st[cur];
+  const PathState &pn = a.st[cur ^ 1];
+  const size_t P = a.capacity;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t i = base + threadIdx.x;
     bool cont = false, want_mis = false;
-    uint32_t slot = 0;
-    V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0};
+    uint32_t nshadow = 0, js = 0, pixel = 0, sample = 0, ctr = 0;
+    uint64_t vmask = 0;  // valid shadow rays k < 64
+    int K = 0, depth = 0;
+    V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0}, T{0, 0, 0};
     if (i < n) {
       const float4 qo = q[2 * i], qd = q[2 * i + 1];
-      slot = __float_as_uint(qo.w);
-      const JobDev &job = a.jobs[a.slot_job[slot]];
-      const DevSceneView &s = a.scenes[job.scene];
+      js = ps.job[i];
+      const JobDev job = a.jobs[js];
+      const DevSceneView s = a.scenes[job.scene];
       const int32_t prim = a.hit_prim[i];
-      const int depth = a.depth[slot];
-      for (int k = 0; k < job.K; ++k) {
-        a.s_vis[size_t(slot) * kmax + k] = 0;
-        a.s_valid[size_t(slot) * kmax + k] = 0;
+      depth = ps.depth[i];
+      K = job.K;
+      for (int k = 0; k < K; ++k) {
+        a.s_vis[k * P + i] = 0;
+        a.s_valid[k * P + i] = 0;
       }
-      a.m_add[3 * slot] = 0.0f;
-      a.m_add[3 * slot + 1] = 0.0f;
-      a.m_add[3 * slot + 2] = 0.0f;
+      a.m_add[3 * i] = 0.0f;
+      a.m_add[3 * i + 1] = 0.0f;
+      a.m_add[3 * i + 2] = 0.0f;
       if (prim >= 0) {
         const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
         const HitRecord h = hit_record(s, uint32_t(prim), ro, rd);
         const Mat m = load_mat(s.materials + size_t(s.prim_mat[prim]) * kMatStride);
-        const uint32_t key = path_key(job.seed, job.key, a.slot_pixel[slot], a.slot_sample[slot]);
-        uint32_t ctr = a.ctr[slot];
-        V3 T{a.thr[3 * slot], a.thr[3 * slot + 1], a.thr[3 * slot + 2]};
-        V3 rad{a.rad[3 * slot], a.rad[3 * slot + 1], a.rad[3 * slot + 2]};
+        pixel = ps.pixel[i];
+        sample = ps.sample[i];
+        const uint32_t key = path_key(job.seed, job.key, pixel, sample);
+        ctr = ps.ctr[i];
+        T = V3{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
+        V3 rad{ps.rad[3 * i], ps.rad[3 * i + 1], ps.rad[3 * i + 2]};
         shift = h.pos + kEps * h.normal;
+        a.shift[i] = make_float4(shift.x, shift.y, shift.z, 0.0f);
         if (depth >= job.min_depth) {
           // emission (muon/integration.cc:257-264)
           if (!((job.nee != MUONX_NEE_OFF && depth > 0) || dot(h.normal, -rd) < 0.0f)) rad = rad + T * m.emission;
@@ -128,13 +155,16 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
                 if (ldist < __builtin_inff()) irr = irr / (ldist * ldist);
                 const float cosi = gmax(dot(h.normal, ldir), 0.0f);
                 const V3 contrib = T * (irr * cosi * brdf_eval(m, ldir, rd, h.normal));
-                const size_t sl = size_t(slot) * kmax + k;
+                const size_t sl = k * P + i;
                 a.s_contrib[3 * sl] = contrib.x;
                 a.s_contrib[3 * sl + 1] = contrib.y;
                 a.s_contrib[3 * sl + 2] = contrib.z;
                 a.s_dir[sl] = make_float4(ldir.x, ldir.y, ldir.z, ldist);
-                a.s_org[sl] = make_float4(shift.x, shift.y, shift.z, 0.0f);
-                a.s_valid[sl] = (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) ? 1 : 0;
+                // a zero contribution cannot change the image: its ray is skipped
+                const uint8_t valid = (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) ? 1 : 0;
+                a.s_valid[sl] = valid;
+                nshadow += valid;
+                if (valid && k < 64) vmask |= 1ull << k;
                 ++k;
                 continue;
               }
@@ -162,14 +192,15 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
                       c = c * power_heuristic(lpdf, bpdf);
                     }
                     const V3 contrib = T * (ql.color * (c * scale));
-                    const size_t sl = size_t(slot) * kmax + k;
+                    const size_t sl = k * P + i;
                     a.s_contrib[3 * sl] = contrib.x;
                     a.s_contrib[3 * sl + 1] = contrib.y;
                     a.s_contrib[3 * sl + 2] = contrib.z;
                     a.s_dir[sl] = make_float4(ldir.x, ldir.y, ldir.z, ldist);
-                    a.s_org[sl] = make_float4(shift.x, shift.y, shift.z, 0.0f);
-                    // a zero contribution cannot change the image: its ray is skipped
-                    a.s_valid[sl] = (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) ? 1 : 0;
+                    const uint8_t valid = (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) ? 1 : 0;
+                    a.s_valid[sl] = valid;
+                    nshadow += valid;
+                    if (valid && k < 64) vmask |= 1ull << k;
                     ++k;
                   }
                 }
@@ -182,9 +213,9 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
               if (sample_reflection(job.is_mode, m, rd, h.normal, T, key, ctr, dir, bpdf, nT)) {
                 const float lpdf = nee_pdf(s, shift, dir);
                 const V3 f = nT * power_heuristic(bpdf, lpdf);
-                a.m_fac[3 * slot] = f.x;
-                a.m_fac[3 * slot + 1] = f.y;
-                a.m_fac[3 * slot + 2] = f.z;
+                a.m_fac[3 * i] = f.x;
+                a.m_fac[3 * i + 1] = f.y;
+                a.m_fac[3 * i + 2] = f.z;
                 want_mis = true;
                 mis_dir = dir;
               }
@@ -208,86 +239,95 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
             T = nT;
           }
         }
-        a.ctr[slot] = ctr;
-        a.thr[3 * slot] = T.x;
-        a.thr[3 * slot + 1] = T.y;
-        a.thr[3 * slot + 2] = T.z;
-        a.rad[3 * slot] = rad.x;
-        a.rad[3 * slot + 1] = rad.y;
-        a.rad[3 * slot + 2] = rad.z;
+        ps.rad[3 * i] = rad.x;
+        ps.rad[3 * i + 1] = rad.y;
+        ps.rad[3 * i + 2] = rad.z;
       }
-      a.alive[slot] = cont ? 1 : 0;
-      if (cont) a.depth[slot] = depth + 1;
     }
-    const uint32_t ni = wave_append(cont, next_count);
+    // a continuing path moves to the next queue with its state; its radiance
+    // follows in retire, after the NEE / MIS terms are added
+    const uint32_t ni = wave_append(cont, &a.c->qcount[cur ^ 1]);
+    if (i < n) a.nidx[i] = cont ? ni : 0xffffffffu;
     if (cont) {
-      qn[2 * ni] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(slot));
+      qn[2 * ni] = make_float4(shift.x, shift.y, shift.z, 0.0f);
       qn[2 * ni + 1] = make_float4(next_dir.x, next_dir.y, next_dir.z, 0.0f);
+      pn.job[ni] = js;
+      pn.pixel[ni] = pixel;
+      pn.sample[ni] = sample;
+      pn.ctr[ni] = ctr;
+      pn.depth[ni] = depth + 1;
+      pn.thr[3 * ni] = T.x;
+      pn.thr[3 * ni + 1] = T.y;
+      pn.thr[3 * ni + 2] = T.z;
     }
     const uint32_t mi = wave_append(want_mis, &a.c->mis_count);
     if (want_mis) {
-      a.m_ray[2 * mi] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(slot));
+      a.m_ray[2 * mi] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(i));
       a.m_ray[2 * mi + 1] = make_float4(mis_dir.x, mis_dir.y, mis_dir.z, 0.0f);
+    }
+    // the shadow rays to trace, compacted (zero contributions are skipped)
+    uint32_t si = wave_append_n(nshadow, &a.c->shadow_count);
+    for (uint64_t m = vmask; m; m &= m - 1) a.s_queue[si++] = uint32_t(size_t(__ffsll((unsigned long long)m) - 1) * P + i);
+    for (int k = 64; nshadow && k < K; ++k) {
+      const uint32_t sl = uint32_t(k * P + i);
+      if (a.s_valid[sl]) a.s_queue[si++] = sl;
     }
   }
 }
 
 // ---------------------------------------------------------------- shadow (any hit)
-template <int KIND, bool WW>
+template <int KIND, bool WW, int WIDTH>
 __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
-  const uint32_t kmax = uint32_t(a.kmax);
-  const uint32_t n = a.c->qcount[cur] * kmax;
-  const float4 *q = a.q_ray[cur];
+  const uint32_t P = a.capacity;
+  const uint32_t n = a.c->shadow_count;
+  const PathState &ps = a.st[cur];
   uint32_t traced = 0;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
-    const uint32_t i = base + threadIdx.x;
-    if (i < n) {
-      const uint32_t e = i / kmax, k = i % kmax;
-      const uint32_t slot = __float_as_uint(q[2 * e].w);
-      const JobDev &job = a.jobs[a.slot_job[slot]];
-      const size_t sl = size_t(slot) * kmax + k;
-      if (int(k) < job.K && a.hit_prim[e] >= 0 && a.s_valid[sl]) {
-        const float4 so = a.s_org[sl], sd = a.s_dir[sl];
-        const TraceRay r = make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z});
-        float best = sd.w;
-        TraceCounters c;
-        auto st = make_stack<KIND>(lds, kEngBlock, sa);
-        const int32_t hit = trace_bvh_s<true, false, false, WW>(a.scenes[job.scene], r, best, st, c);
-        a.s_vis[sl] = hit < 0 ? 1 : 0;
-        ++traced;
-      }
+    const uint32_t j = base + threadIdx.x;
+    if (j < n) {
+      const uint32_t sl = a.s_queue[j];
+      const uint32_t i = sl % P;
+      const DevSceneView sv = a.scenes[a.jobs[ps.job[i]].scene];
+      const float4 so = a.shift[i], sd = a.s_dir[sl];
+      const TraceRay r = make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z});
+      float best = sd.w;
+      TraceCounters c;
+      auto st = make_stack<KIND>(lds, kEngBlock, sa);
+      const int32_t hit = trace_fast<WIDTH, true, false, WW>(sv, r, best, st, c);
+      a.s_vis[sl] = hit < 0 ? 1 : 0;
+      ++traced;
     }
   }
   wave_count(traced, &a.c->rays[2]);
 }
 
 // ---------------------------------------------------------------- MIS extend
-template <int KIND, bool WW>
-__global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa) {
+template <int KIND, bool WW, int WIDTH>
+__global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t n = a.c->mis_count;
+  const PathState &ps = a.st[cur];
   uint32_t traced = 0;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
-    const uint32_t i = base + threadIdx.x;
-    if (i < n) {
-      const float4 mo = a.m_ray[2 * i], md = a.m_ray[2 * i + 1];
-      const uint32_t slot = __float_as_uint(mo.w);
-      const JobDev &job = a.jobs[a.slot_job[slot]];
-      const DevSceneView &s = a.scenes[job.scene];
+    const uint32_t j = base + threadIdx.x;
+    if (j < n) {
+      const float4 mo = a.m_ray[2 * j], md = a.m_ray[2 * j + 1];
+      const uint32_t i = __float_as_uint(mo.w);
+      const DevSceneView s = a.scenes[a.jobs[ps.job[i]].scene];
       const TraceRay r = make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z});
       float best = __builtin_inff();
       TraceCounters c;
       auto st = make_stack<KIND>(lds, kEngBlock, sa);
-      const int32_t hit = trace_bvh_s<false, false, false, WW>(s, r, best, st, c);
+      const int32_t hit = trace_fast<WIDTH, false, false, WW>(s, r, best, st, c);
       ++traced;
       if (hit >= 0) {
         const int32_t light = s.prim_light[hit];
         if (light >= 0) {
           const float *L = s.lights + size_t(light) * kLightStride;
-          a.m_add[3 * slot] = a.m_fac[3 * slot] * L[1];
-          a.m_add[3 * slot + 1] = a.m_fac[3 * slot + 1] * L[2];
-          a.m_add[3 * slot + 2] = a.m_fac[3 * slot + 2] * L[3];
+          a.m_add[3 * i] = a.m_fac[3 * i] * L[1];
+          a.m_add[3 * i + 1] = a.m_fac[3 * i + 1] * L[2];
+          a.m_add[3 * i + 2] = a.m_fac[3 * i + 2] * L[3];
         }
       }
     }
@@ -296,112 +336,118 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa) 
 }
 
 // ---------------------------------------------------------------- retire
-// Adds each shaded path's NEE and MIS terms in a fixed order; finished paths
-// add their radiance to the job's film and free their slot.
+// Adds each shaded path's NEE and MIS terms in a fixed order. A continuing
+// path's radiance moves to its place in the next queue; a finished path adds
+// it to the job's film. Done counters take one atomic per job per wavefront.
 __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
   const uint32_t n = a.c->qcount[cur];
-  const float4 *q = a.q_ray[cur];
-  const int kmax = a.kmax;
-  for (uint32_t i = blockIdx.x * kEngBlock + threadIdx.x; i < n; i += gridDim.x * kEngBlock) {
-    const uint32_t slot = __float_as_uint(q[2 * i].w);
-    const JobDev &job = a.jobs[a.slot_job[slot]];
-    V3 rad{a.rad[3 * slot], a.rad[3 * slot + 1], a.rad[3 * slot + 2]};
-    if (a.hit_prim[i] >= 0) {
-      V3 direct{0.0f, 0.0f, 0.0f};
-      for (int k = 0; k < job.K; ++k) {
-        const size_t sl = size_t(slot) * kmax + k;
-        if (a.s_vis[sl]) direct = direct + V3{a.s_contrib[3 * sl], a.s_contrib[3 * sl + 1], a.s_contrib[3 * sl + 2]};
+  const PathState &ps = a.st[cur];
+  const PathState &pn = a.st[cur ^ 1];
+  const size_t P = a.capacity;
+  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+    const uint32_t i = base + threadIdx.x;
+    bool fin = false;
+    uint32_t js = 0xffffffffu;
+    if (i < n) {
+      js = ps.job[i];
+      V3 rad{ps.rad[3 * i], ps.rad[3 * i + 1], ps.rad[3 * i + 2]};
+      if (a.hit_prim[i] >= 0) {
+        const int K = a.jobs[js].K;
+        V3 direct{0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < K; ++k) {
+          const size_t sl = k * P + i;
+          if (a.s_vis[sl]) direct = direct + V3{a.s_contrib[3 * sl], a.s_contrib[3 * sl + 1], a.s_contrib[3 * sl + 2]};
+        }
+        direct = direct + V3{a.m_add[3 * i], a.m_add[3 * i + 1], a.m_add[3 * i + 2]};
+        rad = rad + direct;
       }
-      direct = direct + V3{a.m_add[3 * slot], a.m_add[3 * slot + 1], a.m_add[3 * slot + 2]};
-      rad = rad + direct;
-      a.rad[3 * slot] = rad.x;
-      a.rad[3 * slot + 1] = rad.y;
-      a.rad[3 * slot + 2] = rad.z;
+      const uint32_t ni = a.nidx[i];
+      if (ni != 0xffffffffu) {
+        pn.rad[3 * ni] = rad.x;
+        pn.rad[3 * ni + 1] = rad.y;
+        pn.rad[3 * ni + 2] = rad.z;
+      } else {
+        unsigned long long *px = a.jobs[js].film + 3 * size_t(ps.pixel[i]);
+        const unsigned long long fx = to_fixed(rad.x), fy = to_fixed(rad.y), fz = to_fixed(rad.z);
+        if (fx) atomicAdd(px, fx);
+        if (fy) atomicAdd(px + 1, fy);
+        if (fz) atomicAdd(px + 2, fz);
+        fin = true;
+      }
     }
-    if (!a.alive[slot]) {
-      unsigned long long *px = job.film + 3 * size_t(a.slot_pixel[slot]);
-      const unsigned long long fx = to_fixed(rad.x), fy = to_fixed(rad.y), fz = to_fixed(rad.z);
-      if (fx) atomicAdd(px, fx);
-      if (fy) atomicAdd(px + 1, fy);
-      if (fz) atomicAdd(px + 2, fz);
-      atomicAdd(job.done, 1ull);
-      a.free_list[atomicAdd(&a.c->free_count, 1u)] = slot;
+    uint64_t pending = __ballot(fin);
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const uint32_t jl = uint32_t(__shfl(int(js), leader));
+      const uint64_t same = __ballot(fin && js == jl);
+      if (int(__lane_id()) == leader) atomicAdd(a.jobs[jl].done, (unsigned long long)__popcll(same));
+      pending &= ~same;
     }
   }
 }
 
 // ---------------------------------------------------------------- plan / fill
+// After retire: the queue just consumed is empty; new samples go behind the
+// continuing paths of the next queue.
 __global__ void k_eng_plan(EngArgs a, int cur) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   EngCounters *c = a.c;
   c->qcount[cur] = 0;
   c->mis_count = 0;
+  c->shadow_count = 0;
+  const uint32_t next = c->qcount[cur ^ 1];
   const unsigned long long avail = c->total_items > c->cursor ? c->total_items - c->cursor : 0ull;
-  const uint32_t nfill = uint32_t(avail < c->free_count ? avail : c->free_count);
+  const uint32_t room = a.capacity - next;
+  const uint32_t nfill = uint32_t(avail < room ? avail : room);
   c->plan_n = nfill;
-  c->plan_free_base = c->free_count - nfill;
+  c->plan_base = next;
   c->plan_cursor = c->cursor;
-  c->free_count -= nfill;
   c->cursor += nfill;
+  c->qcount[cur ^ 1] = next + nfill;
 }
 
 __global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
   const EngCounters *c = a.c;
   const uint32_t n = c->plan_n;
-  const uint32_t free_base = c->plan_free_base;
+  const uint32_t qbase = c->plan_base;
   const unsigned long long cursor = c->plan_cursor;
   const unsigned long long lo = c->seq_lo, hi = c->seq_hi;
-  uint32_t *next_count = &a.c->qcount[cur ^ 1];
   float4 *qn = a.q_ray[cur ^ 1];
-  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
-    const uint32_t i = base + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t slot = 0;
-    V3 o{0, 0, 0}, d{0, 0, 0};
-    if (valid) {
-      slot = a.free_list[free_base + i];
-      const unsigned long long item = cursor + i;
-      // last admitted job whose first item is <= item
-      unsigned long long l = lo, h = hi;  // invariant: job_first[l] <= item < job_first[h]
-      while (h - l > 1) {
-        const unsigned long long m = l + (h - l) / 2;
-        if (a.job_first[m % a.max_jobs] <= item) l = m;
-        else h = m;
-      }
-      const uint32_t js = uint32_t(l % a.max_jobs);
-      const JobDev &job = a.jobs[js];
-      const unsigned long long local = item - job.first_item;
-      const uint32_t sample = uint32_t(local / job.npix);
-      const uint32_t pixel = uint32_t(local % job.npix);
-      a.slot_job[slot] = js;
-      a.slot_pixel[slot] = pixel;
-      a.slot_sample[slot] = sample;
-      a.depth[slot] = 0;
-      a.ctr[slot] = 0;
-      a.alive[slot] = 1;
-      a.thr[3 * slot] = 1.0f;
-      a.thr[3 * slot + 1] = 1.0f;
-      a.thr[3 * slot + 2] = 1.0f;
-      a.rad[3 * slot] = 0.0f;
-      a.rad[3 * slot + 1] = 0.0f;
-      a.rad[3 * slot + 2] = 0.0f;
-      const int px = int(pixel % uint32_t(job.cam.width)), py = int(pixel / uint32_t(job.cam.width));
-      float x, y;
-      sample_position(px, py, int(sample), path_key(job.seed, job.key, pixel, sample), x, y);
-      cast_ray(job.cam, x, y, o, d);
+  const PathState &pn = a.st[cur ^ 1];
+  for (uint32_t j = blockIdx.x * kEngBlock + threadIdx.x; j < n; j += gridDim.x * kEngBlock) {
+    const unsigned long long item = cursor + j;
+    // last admitted job whose first item is <= item
+    unsigned long long l = lo, h = hi;  // invariant: job_first[l] <= item < job_first[h]
+    while (h - l > 1) {
+      const unsigned long long m = l + (h - l) / 2;
+      if (a.job_first[m % a.max_jobs] <= item) l = m;
+      else h = m;
     }
-    const uint32_t qi = wave_append(valid, next_count);
-    if (valid) {
-      qn[2 * qi] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
-      qn[2 * qi + 1] = make_float4(d.x, d.y, d.z, 0.0f);
-    }
+    const uint32_t js = uint32_t(l % a.max_jobs);
+    const JobDev &job = a.jobs[js];
+    const unsigned long long local = item - job.first_item;
+    const uint32_t sample = uint32_t(local / job.npix);
+    const uint32_t pixel = uint32_t(local % job.npix);
+    const int px = int(pixel % uint32_t(job.cam.width)), py = int(pixel / uint32_t(job.cam.width));
+    float x, y;
+    sample_position(px, py, int(sample), path_key(job.seed, job.key, pixel, sample), x, y);
+    V3 o, d;
+    cast_ray(job.cam, x, y, o, d);
+    const uint32_t ni = qbase + j;
+    pn.job[ni] = js;
+    pn.pixel[ni] = pixel;
+    pn.sample[ni] = sample;
+    pn.ctr[ni] = 0;
+    pn.depth[ni] = 0;
+    pn.thr[3 * ni] = 1.0f;
+    pn.thr[3 * ni + 1] = 1.0f;
+    pn.thr[3 * ni + 2] = 1.0f;
+    pn.rad[3 * ni] = 0.0f;
+    pn.rad[3 * ni + 1] = 0.0f;
+    pn.rad[3 * ni + 2] = 0.0f;
+    qn[2 * ni] = make_float4(o.x, o.y, o.z, 0.0f);
+    qn[2 * ni + 1] = make_float4(d.x, d.y, d.z, 0.0f);
   }
-}
-
-__global__ void k_eng_init(EngArgs a) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.capacity; i += gridDim.x * blockDim.x)
-    a.free_list[i] = a.capacity - 1 - i;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.c->free_count = a.capacity;
 }
 
 __global__ void k_film_resolve(const unsigned long long *__restrict__ film, uint32_t n, float *__restrict__ out) {
@@ -421,38 +467,35 @@ T *Carve(uintptr_t &p, size_t count) {
   return r;
 }
 
-size_t CarveSlots(uintptr_t base, size_t P, EngArgs &a) {
+size_t CarvePaths(uintptr_t base, size_t P, EngArgs &a) {
   uintptr_t p = base;
-  a.slot_job = Carve<uint32_t>(p, P);
-  a.slot_pixel = Carve<uint32_t>(p, P);
-  a.slot_sample = Carve<uint32_t>(p, P);
-  a.ctr = Carve<uint32_t>(p, P);
-  a.free_list = Carve<uint32_t>(p, P);
-  a.depth = Carve<int32_t>(p, P);
-  a.alive = Carve<uint8_t>(p, P);
-  a.rad = Carve<float>(p, 3 * P);
-  a.thr = Carve<float>(p, 3 * P);
-  a.m_fac = Carve<float>(p, 3 * P);
-  a.m_add = Carve<float>(p, 3 * P);
-  return size_t(p - base);
-}
-
-size_t CarveQueues(uintptr_t base, size_t P, EngArgs &a) {
-  uintptr_t p = base;
+  for (auto &st : a.st) {
+    st.job = Carve<uint32_t>(p, P);
+    st.pixel = Carve<uint32_t>(p, P);
+    st.sample = Carve<uint32_t>(p, P);
+    st.ctr = Carve<uint32_t>(p, P);
+    st.depth = Carve<int32_t>(p, P);
+    st.rad = Carve<float>(p, 3 * P);
+    st.thr = Carve<float>(p, 3 * P);
+  }
   a.q_ray[0] = Carve<float4>(p, 2 * P);
   a.q_ray[1] = Carve<float4>(p, 2 * P);
   a.m_ray = Carve<float4>(p, 2 * P);
   a.hit_prim = Carve<int32_t>(p, P);
+  a.nidx = Carve<uint32_t>(p, P);
+  a.m_fac = Carve<float>(p, 3 * P);
+  a.m_add = Carve<float>(p, 3 * P);
+  a.shift = Carve<float4>(p, P);
   return size_t(p - base);
 }
 
 size_t CarveShadow(uintptr_t base, size_t P, size_t K, EngArgs &a) {
   uintptr_t p = base;
-  a.s_org = Carve<float4>(p, P * K);
   a.s_dir = Carve<float4>(p, P * K);
   a.s_contrib = Carve<float>(p, 3 * P * K);
   a.s_vis = Carve<uint8_t>(p, P * K);
   a.s_valid = Carve<uint8_t>(p, P * K);
+  a.s_queue = Carve<uint32_t>(p, P * K);
   return size_t(p - base);
 }
 
@@ -461,14 +504,14 @@ struct IterLaunch {
   const TraverseCfg &cfg;
   int cur;
   hipStream_t st;
-  template <int KIND, bool WW>
+  template <int KIND, bool WW, int WIDTH>
   void run() const {
     const StackArgs sa{cfg.spill, cfg.gcap};
     const dim3 grid(cfg.grid), block(kEngBlock);
-    hipLaunchKernelGGL((k_eng_extend<KIND, WW>), grid, block, 0, st, a, sa, cur);
+    hipLaunchKernelGGL((k_eng_extend<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
     hipLaunchKernelGGL(k_eng_shade, grid, block, 0, st, a, cur);
-    hipLaunchKernelGGL((k_eng_shadow<KIND, WW>), grid, block, 0, st, a, sa, cur);
-    hipLaunchKernelGGL((k_eng_mis<KIND, WW>), grid, block, 0, st, a, sa);
+    hipLaunchKernelGGL((k_eng_shadow<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
+    hipLaunchKernelGGL((k_eng_mis<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
     hipLaunchKernelGGL(k_eng_retire, grid, block, 0, st, a, cur);
     hipLaunchKernelGGL(k_eng_plan, dim3(1), dim3(64), 0, st, a, cur);
     hipLaunchKernelGGL(k_eng_fill, grid, block, 0, st, a, cur);
@@ -485,7 +528,7 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
     return MUONX_ERR_INVALID;
   }
   if (capacity > (1u << 30) || uint64_t(capacity) * uint64_t(kmax) > (uint64_t(1) << 31)) {
-    SetError("engine: path slots x shadow rays per vertex must stay below 2^31");
+    SetError("engine: paths in flight x shadow rays per vertex must stay below 2^31");
     return MUONX_ERR_INVALID;
   }
   Engine *e = new Engine();
@@ -506,25 +549,23 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
   }
   // The buffers are carved twice: from a null base to size them, then for real.
   EngArgs &a = e->a_;
-  size_t slot_bytes = 0, queue_bytes = 0, shadow_bytes = 0;
+  size_t path_bytes = 0, shadow_bytes = 0;
   for (int pass = 0; pass < 2; ++pass) {
-    const uintptr_t b0 = pass ? reinterpret_cast<uintptr_t>(e->slots_.ptr) : 0;
-    const uintptr_t b1 = pass ? reinterpret_cast<uintptr_t>(e->queues_.ptr) : 0;
-    const uintptr_t b2 = pass ? reinterpret_cast<uintptr_t>(e->shadow_.ptr) : 0;
-    slot_bytes = CarveSlots(b0, capacity, a);
-    queue_bytes = CarveQueues(b1, capacity, a);
-    shadow_bytes = CarveShadow(b2, capacity, size_t(kmax), a);
+    const uintptr_t b0 = pass ? reinterpret_cast<uintptr_t>(e->paths_.ptr) : 0;
+    const uintptr_t b1 = pass ? reinterpret_cast<uintptr_t>(e->shadow_.ptr) : 0;
+    path_bytes = CarvePaths(b0, capacity, a);
+    shadow_bytes = CarveShadow(b1, capacity, size_t(kmax), a);
     if (pass == 0 &&
-        (e->slots_.Alloc(slot_bytes) != hipSuccess || e->queues_.Alloc(queue_bytes) != hipSuccess ||
-         e->shadow_.Alloc(shadow_bytes) != hipSuccess || e->counters_.Alloc(sizeof(EngCounters)) != hipSuccess ||
+        (e->paths_.Alloc(path_bytes) != hipSuccess || e->shadow_.Alloc(shadow_bytes) != hipSuccess ||
+         e->counters_.Alloc(sizeof(EngCounters)) != hipSuccess ||
          e->jobs_.Alloc(sizeof(JobDev) * max_jobs) != hipSuccess ||
          e->job_first_.Alloc(sizeof(unsigned long long) * max_jobs) != hipSuccess ||
          e->done_.Alloc(sizeof(unsigned long long) * max_jobs) != hipSuccess)) {
-      SetError("engine: out of device memory for " + std::to_string(capacity) + " path slots");
+      SetError("engine: out of device memory for " + std::to_string(capacity) + " paths");
       return fail(MUONX_ERR_NOMEM);
     }
   }
-  e->device_bytes_ = slot_bytes + queue_bytes + shadow_bytes;
+  e->device_bytes_ = path_bytes + shadow_bytes;
   if (hipHostMalloc(reinterpret_cast<void **>(&e->h_jobs_), sizeof(JobDev) * max_jobs) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&e->h_first_), sizeof(unsigned long long) * max_jobs) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&e->h_window_), sizeof(unsigned long long) * 3 * 4) != hipSuccess ||
@@ -547,8 +588,6 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
   if (hipMemsetAsync(a.c, 0, sizeof(EngCounters), e->stream_) != hipSuccess ||
       hipMemsetAsync(e->done_.ptr, 0, sizeof(unsigned long long) * max_jobs, e->stream_) != hipSuccess)
     return fail(MUONX_ERR_HIP);
-  hipLaunchKernelGGL(k_eng_init, dim3(256), dim3(256), 0, e->stream_, a);
-  if (hipGetLastError() != hipSuccess) return fail(MUONX_ERR_HIP);
   *out = e;
   return MUONX_OK;
 }
@@ -603,11 +642,13 @@ int Engine::AddScene(muonx_scene *scene, uint32_t *slot) {
   *slot = uint32_t(at);
   // the traversal plan covers the deepest resident BVH
   const uint32_t depth = std::max(max_depth_, scene->view.bvh_depth);
-  if (depth != max_depth_ || !cfg_.grid) {
+  const uint32_t wstack = std::max(max_wstack_, scene->view.wstack);
+  if (depth != max_depth_ || wstack != max_wstack_ || !cfg_.grid) {
     max_depth_ = depth;
+    max_wstack_ = wstack;
     TraverseCfg cfg;
-    const int rc =
-        PlanTraverseInto(tuning_, num_cus_, max_depth_, uint64_t(capacity_) * kmax_, kEngBlock, spill_, cfg, false);
+    const int rc = PlanTraverseInto(tuning_, num_cus_, max_depth_, max_wstack_, uint64_t(capacity_) * kmax_,
+                                    kEngBlock, spill_, cfg, false);
     if (rc != MUONX_OK) return rc;
     cfg_ = cfg;
     for (auto &g : graph_) {

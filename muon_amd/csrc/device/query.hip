@@ -7,7 +7,7 @@
 
 namespace muonx {
 
-template <int KIND, bool WW, bool ANY, bool COUNT, bool EXACT>
+template <int KIND, bool WW, int WIDTH, bool ANY, bool COUNT, bool EXACT>
 __global__ void __launch_bounds__(kQueryBlock) k_query(DevSceneView s, StackArgs sa, const float *__restrict__ ro,
                                                        const float *__restrict__ rd, const float *__restrict__ tmax,
                                                        uint32_t n, int32_t *__restrict__ prim_out,
@@ -21,7 +21,9 @@ __global__ void __launch_bounds__(kQueryBlock) k_query(DevSceneView s, StackArgs
     float best = tmax ? tmax[i] : __builtin_inff();
     const TraceRay r = make_ray(o, d);
     auto st = make_stack<KIND>(lds, kQueryBlock, sa);
-    const int32_t p = trace_bvh_s<ANY, COUNT, EXACT, WW>(s, r, best, st, cnt);
+    int32_t p;
+    if constexpr (EXACT) p = trace_bvh_s<ANY, COUNT, true, false>(s, r, best, st, cnt);
+    else p = trace_fast<WIDTH, ANY, COUNT, WW>(s, r, best, st, cnt);
     if (ANY) {
       occ_out[i] = p >= 0 ? 1 : 0;
     } else {
@@ -55,11 +57,11 @@ struct QueryLaunch {
   uint8_t *occ;
   unsigned long long *counters;
   hipStream_t stream;
-  template <int KIND, bool WW>
+  template <int KIND, bool WW, int WIDTH>
   void run() const {
     const StackArgs sa{cfg.spill, cfg.gcap};
-    hipLaunchKernelGGL((k_query<KIND, WW, ANY, COUNT, EXACT>), dim3(cfg.grid), dim3(kQueryBlock), 0, stream, s, sa,
-                       ro, rd, tmax, n, prim, t, occ, counters);
+    hipLaunchKernelGGL((k_query<KIND, WW, WIDTH, ANY, COUNT, EXACT>), dim3(cfg.grid), dim3(kQueryBlock), 0, stream,
+                       s, sa, ro, rd, tmax, n, prim, t, occ, counters);
   }
 };
 }  // namespace
@@ -69,16 +71,17 @@ int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, con
                 hipStream_t stream) {
   if (n == 0) return MUONX_OK;
   TraverseCfg cfg;
-  int rc = PlanTraverse(ctx, s.bvh_depth, n, kQueryBlock, exact, cfg);
+  int rc = PlanTraverse(ctx, s, n, kQueryBlock, exact, cfg);
   if (rc != MUONX_OK) return rc;
   if ((rc = BeginSpill(ctx, cfg, stream)) != MUONX_OK) return rc;
 #define MX_Q(A, C, E) DispatchTraverse(cfg, QueryLaunch<A, C, E>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, stream})
+#define MX_QE(A, C) DispatchExact(cfg, QueryLaunch<A, C, true>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, stream})
   const bool cnt = counters != nullptr;
   if (exact) {
     // exact mode: full LDS stack, one node or leaf step per iteration
-    if (any) MX_Q(true, false, true);
-    else if (cnt) MX_Q(false, true, true);
-    else MX_Q(false, false, true);
+    if (any) MX_QE(true, false);
+    else if (cnt) MX_QE(false, true);
+    else MX_QE(false, false);
   } else if (any) {
     if (cnt) MX_Q(true, true, false);
     else MX_Q(true, false, false);
@@ -87,6 +90,7 @@ int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, con
     else MX_Q(false, false, false);
   }
 #undef MX_Q
+#undef MX_QE
   MX_HIP(hipGetLastError());
   return EndSpill(ctx, cfg, stream);
 }

@@ -13,6 +13,7 @@
 namespace muonx {
 
 void DestroyWorkspace(RenderWorkspace *ws) {
+  if (!ws) return;  // the context never rendered
   DestroyContextScheduler(ws);
   if (ws->ev_in) (void)hipEventDestroy(ws->ev_in);
   if (ws->ev_out) (void)hipEventDestroy(ws->ev_out);

@@ -32,6 +32,87 @@ float Reflectiveness(const float *m) {
   return denom > 0.0f ? s / denom : 0.0f;
 }
 
+float SurfaceArea(const float *b) {
+  const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+// Collapses the binary tree (pre-order arrays of the descriptor) into 4-ary
+// nodes: each wide node starts from a binary node's two children and keeps
+// replacing the internal child of largest surface area by its two children
+// while it has fewer than 4. Leaves stay the binary leaves. Returns the
+// nodes (8 float4 each, pre-order) and in `max_stack` a bound on the stack
+// entries a traversal can hold: the largest sum of (children - 1) along a
+// root-to-leaf path.
+std::vector<float4> CollapseWide(const muonx_scene_desc *d, uint32_t *max_stack) {
+  std::vector<float4> out;
+  struct Item {
+    int32_t bin;     // binary internal node
+    uint32_t wide;   // its wide node index
+    uint32_t pushes; // stack entries held above it
+  };
+  std::vector<Item> todo;
+  out.resize(8);
+  todo.push_back({0, 0, 0});
+  *max_stack = 1;
+  while (!todo.empty()) {
+    const Item it = todo.back();
+    todo.pop_back();
+    int32_t c[4] = {d->node_left[it.bin], d->node_right[it.bin], -1, -1};
+    int n = 2;
+    while (n < 4) {
+      int best = -1;
+      float best_sa = -1.0f;
+      for (int k = 0; k < n; ++k) {
+        if (d->node_count[c[k]] != 0) continue;  // leaf
+        const float sa = SurfaceArea(d->node_bounds + size_t(c[k]) * 6);
+        if (sa > best_sa) {
+          best_sa = sa;
+          best = k;
+        }
+      }
+      if (best < 0) break;
+      const int32_t b = c[best];
+      for (int k = n; k > best + 1; --k) c[k] = c[k - 1];
+      c[best] = d->node_left[b];
+      c[best + 1] = d->node_right[b];
+      ++n;
+    }
+    float lo[3][4], hi[3][4];
+    uint32_t ref[4];
+    const uint32_t pushes = it.pushes + uint32_t(n - 1);
+    for (int k = 0; k < 4; ++k) {
+      if (k >= n) {
+        for (int a = 0; a < 3; ++a) lo[a][k] = hi[a][k] = 0.0f;
+        ref[k] = 0;
+        continue;
+      }
+      const float *b = d->node_bounds + size_t(c[k]) * 6;
+      for (int a = 0; a < 3; ++a) {
+        lo[a][k] = b[a];
+        hi[a][k] = b[a + 3];
+      }
+      if (d->node_count[c[k]] != 0) {
+        ref[k] = kLeafBit | uint32_t(d->node_start[c[k]]);
+        *max_stack = std::max(*max_stack, pushes);
+      } else {
+        const uint32_t w = uint32_t(out.size() / 8);
+        out.resize(out.size() + 8);
+        ref[k] = w;
+        todo.push_back({c[k], w, pushes});
+      }
+    }
+    float4 *q = &out[size_t(it.wide) * 8];
+    for (int a = 0; a < 3; ++a) {
+      q[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+      q[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    q[6] = make_float4(BitsAsFloat(ref[0]), BitsAsFloat(ref[1]), BitsAsFloat(ref[2]), BitsAsFloat(ref[3]));
+    q[7] = make_float4(BitsAsFloat(uint32_t(n)), 0.0f, 0.0f, 0.0f);
+  }
+  return out;
+}
+
 template <typename T>
 int Upload(DevBuf &buf, const std::vector<T> &host, uint64_t &total) {
   MX_HIP(buf.Alloc(host.size() * sizeof(T)));
@@ -198,6 +279,24 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
     }
   }
 
+  // ---- wide BVH (fast traversal mode)
+  std::vector<float4> wnodes;
+  uint32_t wstack = 1;
+  if (root_leaf) {
+    wnodes.assign(8, make_float4(0, 0, 0, 0));
+    const float *b = d->node_bounds;
+    wnodes[0] = make_float4(b[0], 0, 0, 0);
+    wnodes[1] = make_float4(b[3], 0, 0, 0);
+    wnodes[2] = make_float4(b[1], 0, 0, 0);
+    wnodes[3] = make_float4(b[4], 0, 0, 0);
+    wnodes[4] = make_float4(b[2], 0, 0, 0);
+    wnodes[5] = make_float4(b[5], 0, 0, 0);
+    wnodes[6] = make_float4(BitsAsFloat(kLeafBit | uint32_t(d->node_start[0])), 0, 0, 0);
+    wnodes[7] = make_float4(BitsAsFloat(1u), 0, 0, 0);
+  } else {
+    wnodes = CollapseWide(d, &wstack);
+  }
+
   std::vector<float> mats(size_t(d->num_materials) * kMatStride);
   for (uint32_t i = 0; i < d->num_materials; ++i) {
     std::memcpy(&mats[size_t(i) * 16], d->materials + size_t(i) * 16, 16 * sizeof(float));
@@ -221,7 +320,8 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   s->ctx = ctx;
   uint64_t total = 0;
   int rc = MUONX_OK;
-  if ((rc = Upload(s->nodes, nodes, total)) || (rc = Upload(s->prims, prims, total)) ||
+  if ((rc = Upload(s->nodes, nodes, total)) || (rc = Upload(s->wnodes, wnodes, total)) ||
+      (rc = Upload(s->prims, prims, total)) ||
       (rc = Upload(s->xforms, xf, total)) || (rc = Upload(s->prim_id, pid, total)) ||
       (rc = Upload(s->prim_mat, pmat, total)) || (rc = Upload(s->prim_light, plight, total)) ||
       (rc = Upload(s->materials, mats, total)) || (rc = Upload(s->lights, lights, total))) {
@@ -236,6 +336,9 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   }
   DevSceneView &v = s->view;
   v.nodes = s->nodes.as<float4>();
+  v.wnodes = s->wnodes.as<float4>();
+  v.num_wnodes = uint32_t(wnodes.size() / 8);
+  v.wstack = wstack;
   v.prims = s->prims.as<float4>();
   v.xforms = s->xforms.as<float>();
   v.prim_id = s->prim_id.as<int32_t>();

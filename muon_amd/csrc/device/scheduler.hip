@@ -63,9 +63,9 @@ uint64_t FilmBytes(const muonx_render_params &p) {
   return FilmPool::Round(uint64_t(p.width) * uint64_t(p.height) * 3 * sizeof(unsigned long long));
 }
 
-// Device bytes of one path slot (slot state 73, queues 100, 46 per shadow
-// ray), as Engine::Create lays them out.
-uint64_t PathSlotBytes(int kmax) { return 73 + 100 + 46 * uint64_t(kmax); }
+// Device bytes of one path in flight (two queue-ordered states 88, rays and
+// per-path terms 144, 34 per shadow ray), as Engine::Create lays them out.
+uint64_t PathSlotBytes(int kmax) { return 88 + 144 + 34 * uint64_t(kmax); }
 
 }  // namespace
 

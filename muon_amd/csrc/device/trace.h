@@ -343,6 +343,94 @@ __device__ __forceinline__ int32_t trace_bvh_s(const DevSceneView &s, const Trac
   return hit_prim;
 }
 
+// One wide-node step: tests the (up to 4) child boxes with the conservative
+// slab test, moves `ref` to the nearest hit child and pushes the other hit
+// children farthest first (so they pop nearest first). Returns false when no
+// child is hit.
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
+                                           uint32_t &ref, TraceCounters &cnt) {
+  const float4 *np = s.wnodes + 8 * size_t(ref);
+  const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+  const uint4 rf = *reinterpret_cast<const uint4 *>(np + 6);
+  const uint32_t n = __float_as_uint(np[7].x);
+  if (COUNT) cnt.nodes++;
+  constexpr float kInf = __builtin_inff();
+  float t0, t1, t2, t3;
+  const bool h0 = box_hit(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, best, t0);
+  const bool h1 = box_hit(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, best, t1) && n > 1;
+  const bool h2 = box_hit(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best, t2) && n > 2;
+  const bool h3 = box_hit(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best, t3) && n > 3;
+  // (distance, ref) keys; misses sort last
+  float k0 = h0 ? t0 : kInf, k1 = h1 ? t1 : kInf, k2 = h2 ? t2 : kInf, k3 = h3 ? t3 : kInf;
+  uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+  const int nhit = int(h0) + int(h1) + int(h2) + int(h3);
+  if (nhit == 0) return false;
+  auto cswap = [](float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
+    const bool sw = kb < ka;
+    const float tk = sw ? kb : ka;
+    const uint32_t tr = sw ? rb : ra;
+    kb = sw ? ka : kb;
+    rb = sw ? ra : rb;
+    ka = tk;
+    ra = tr;
+  };
+  // sorting network for 4 keys (ascending)
+  cswap(k0, r0, k1, r1);
+  cswap(k2, r2, k3, r3);
+  cswap(k0, r0, k2, r2);
+  cswap(k1, r1, k3, r3);
+  cswap(k1, r1, k2, r2);
+  if (nhit > 3) stack.push(make_uint2(r3, __float_as_uint(k3)));
+  if (nhit > 2) stack.push(make_uint2(r2, __float_as_uint(k2)));
+  if (nhit > 1) stack.push(make_uint2(r1, __float_as_uint(k1)));
+  ref = r0;
+  return true;
+}
+
+// Closest / any hit over the wide BVH (fast mode only: children are visited
+// nearest first, so ties may resolve differently from the reference's order).
+template <bool ANY, bool COUNT, bool WW, class Stack>
+__device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const TraceRay &r, float &best, Stack &stack,
+                                                TraceCounters &cnt) {
+  int32_t hit_prim = -1;
+  ObjCache oc;
+  oc.tid = -1;
+  uint32_t ref = 0;  // root: wide node 0
+  if (WW) {
+    while (true) {
+      bool alive = true;
+      while (!(ref & kLeafBit)) {
+        if (!wnode_step<COUNT>(s, r, best, stack, ref, cnt) && !pop_next(stack, best, ref)) {
+          alive = false;
+          break;
+        }
+      }
+      if (!alive) break;
+      if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
+      if (!pop_next(stack, best, ref)) break;
+    }
+  } else {
+    while (true) {
+      if (!(ref & kLeafBit)) {
+        if (wnode_step<COUNT>(s, r, best, stack, ref, cnt)) continue;
+      } else {
+        if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
+      }
+      if (!pop_next(stack, best, ref)) break;
+    }
+  }
+  return hit_prim;
+}
+
+// Dispatch on the node width (2: binary child-boxes-in-parent, 4: wide).
+template <int WIDTH, bool ANY, bool COUNT, bool WW, class Stack>
+__device__ __forceinline__ int32_t trace_fast(const DevSceneView &s, const TraceRay &r, float &best, Stack &stack,
+                                              TraceCounters &cnt) {
+  if constexpr (WIDTH == 4) return trace_bvh4_s<ANY, COUNT, WW>(s, r, best, stack, cnt);
+  else return trace_bvh_s<ANY, COUNT, false, WW>(s, r, best, stack, cnt);
+}
+
 // Fixed LDS stack at stack[k * stride], k < STACK (STACK >= BVH depth).
 template <int STACK, bool ANY, bool COUNT, bool EXACT = false>
 __device__ __forceinline__ int32_t trace_bvh(const DevSceneView &s, const TraceRay &r, float &best, uint2 *stack,

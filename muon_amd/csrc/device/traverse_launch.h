@@ -15,6 +15,7 @@ namespace muonx {
 struct TraverseCfg {
   int kind;     // 24, 48, 96, 8 or 16
   bool ww;
+  int width;    // 2: binary nodes, 4: wide nodes
   int grid;     // persistent blocks
   int threads;  // grid * block
   uint2 *spill;
@@ -51,19 +52,22 @@ __device__ __forceinline__ typename StackOf<KIND>::T make_stack(uint2 *lds, int 
   }
 }
 
-// Resolves the variant for a BVH depth and reserves the spill region in
-// `spill`. `want_exact` restricts to full LDS stacks (the deterministic
-// modes are not performance-critical and keep the simplest form).
-inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_depth, uint64_t work_items, int block,
-                            DevBuf &spill, TraverseCfg &cfg, bool want_exact = false) {
+// Resolves the variant for a scene (binary depth, wide-stack bound) and
+// reserves the spill region in `spill`. `want_exact` selects the exact mode:
+// binary nodes and full LDS stacks (the deterministic modes are not
+// performance-critical and keep the simplest form).
+inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_depth, uint32_t wstack,
+                            uint64_t work_items, int block, DevBuf &spill, TraverseCfg &cfg, bool want_exact = false) {
   const int cus = num_cus > 0 ? num_cus : 256;
+  cfg.width = (!want_exact && t.bvh_width == 4) ? 4 : 2;
   const int bpc = std::max(1, t.blocks_per_cu);
   cfg.grid = int(std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * bpc, (work_items + block - 1) / block)));
   cfg.threads = cfg.grid * block;
   cfg.ww = t.while_while != 0 && !want_exact;
   cfg.spill = nullptr;
   cfg.gcap = 0;
-  const int need = int(bvh_depth);  // stack entries never exceed depth - 1
+  // binary: entries never exceed depth - 1; wide: the collapse's bound
+  const int need = int(cfg.width == 4 ? std::max<uint32_t>(wstack, 1) : bvh_depth);
   const bool ring = !want_exact && (t.stack_entries == 8 || t.stack_entries == 16);
   if (ring && need <= t.stack_entries) {
     cfg.kind = need <= 8 ? 8 : 16;  // a ring that never spills
@@ -86,9 +90,10 @@ inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_dep
   return MUONX_OK;
 }
 
-inline int PlanTraverse(muonx_ctx *ctx, uint32_t bvh_depth, uint64_t work_items, int block, bool want_exact,
+inline int PlanTraverse(muonx_ctx *ctx, const DevSceneView &s, uint64_t work_items, int block, bool want_exact,
                         TraverseCfg &cfg) {
-  return PlanTraverseInto(ctx->tuning, ctx->num_cus, bvh_depth, work_items, block, ctx->spill, cfg, want_exact);
+  return PlanTraverseInto(ctx->tuning, ctx->num_cus, s.bvh_depth, s.wstack, work_items, block, ctx->spill, cfg,
+                          want_exact);
 }
 
 // Kernels that use the spill region wait for its previous user when issued
@@ -105,15 +110,29 @@ inline int EndSpill(muonx_ctx *ctx, const TraverseCfg &cfg, hipStream_t stream) 
   return MUONX_OK;
 }
 
-// Calls F::template run<KIND, WW>() for the planned variant.
+// Calls F::template run<KIND, WW, WIDTH>() for the planned variant.
+template <int W, class F>
+inline void DispatchKind(const TraverseCfg &cfg, F &&f) {
+  switch (cfg.kind) {
+    case 8: cfg.ww ? f.template run<8, true, W>() : f.template run<8, false, W>(); break;
+    case 16: cfg.ww ? f.template run<16, true, W>() : f.template run<16, false, W>(); break;
+    case 24: cfg.ww ? f.template run<24, true, W>() : f.template run<24, false, W>(); break;
+    case 48: cfg.ww ? f.template run<48, true, W>() : f.template run<48, false, W>(); break;
+    default: cfg.ww ? f.template run<96, true, W>() : f.template run<96, false, W>(); break;
+  }
+}
 template <class F>
 inline void DispatchTraverse(const TraverseCfg &cfg, F &&f) {
+  if (cfg.width == 4) DispatchKind<4>(cfg, f);
+  else DispatchKind<2>(cfg, f);
+}
+// Exact mode: binary nodes, full LDS stack, if-if loop.
+template <class F>
+inline void DispatchExact(const TraverseCfg &cfg, F &&f) {
   switch (cfg.kind) {
-    case 8: cfg.ww ? f.template run<8, true>() : f.template run<8, false>(); break;
-    case 16: cfg.ww ? f.template run<16, true>() : f.template run<16, false>(); break;
-    case 24: cfg.ww ? f.template run<24, true>() : f.template run<24, false>(); break;
-    case 48: cfg.ww ? f.template run<48, true>() : f.template run<48, false>(); break;
-    default: cfg.ww ? f.template run<96, true>() : f.template run<96, false>(); break;
+    case 24: f.template run<24, false, 2>(); break;
+    case 48: f.template run<48, false, 2>(); break;
+    default: f.template run<96, false, 2>(); break;
   }
 }
 

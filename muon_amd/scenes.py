@@ -247,3 +247,59 @@ def microbench_rays(n, lo, hi, seed=0):
     d = np.stack([rr * np.cos(phi), rr * np.sin(phi), z], axis=1).astype(np.float32)
     d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
     return o, d.astype(np.float32)
+
+
+# ---------------------------------------------------------------- cameras and request mixes
+def set_camera(params, eye, look, up=(0.0, 1.0, 0.0), fov_deg=None, width=None, height=None):
+    """Sets the pinhole camera of a muonx_render_params (muon/camera.cc:15-35:
+    w = norm(eye - look), u = norm(up x w), v = w x u, tan(fov/2) on y and
+    scaled by W/H on x). fov_deg None keeps the params' vertical field of view."""
+    if width is not None:
+        params.width = int(width)
+    if height is not None:
+        params.height = int(height)
+    f32 = np.float32
+    e, l, u = (np.asarray(x, f32) for x in (eye, look, up))
+    w = e - l
+    w = w / f32(np.sqrt(np.dot(w, w)))
+    cu = np.cross(u, w).astype(f32)
+    cu = cu / f32(np.sqrt(np.dot(cu, cu)))
+    cv = np.cross(w, cu).astype(f32)
+    tfy = f32(params.tan_fov_y) if fov_deg is None else f32(math.tan(math.radians(fov_deg) / 2.0))
+    params.tan_fov_y = float(tfy)
+    params.tan_fov_x = float(tfy * (f32(params.width) / f32(params.height)))
+    for i in range(3):
+        params.eye[i] = float(e[i])
+        params.cam_u[i] = float(cu[i])
+        params.cam_v[i] = float(cv[i])
+        params.cam_w[i] = float(w[i])
+    params.has_camera = 1
+    return params
+
+
+# Resident scenes for the batched and served workloads: displaced icospheres
+# of levels 3..8 (1,280 .. 1,310,720 tris) in the Cornell enclosure and
+# sphere fields, spanning ~1k to ~1.3M primitives.
+POOL = [("ico", 3), ("ico", 4), ("ico", 5), ("ico", 6), ("ico", 7), ("ico", 8), ("spheres", 3000),
+        ("spheres", 30000)]
+
+
+def pool_scene_text(kind, size, seed=0):
+    if kind == "ico":
+        return icosphere_text(256, 256, 16, 8, level=size, seed=seed)
+    return sphere_field_text(256, 256, 16, 8, n=size, seed=seed)
+
+
+def random_pose(params, kind, rng, width, height):
+    """A seeded camera pose: the eye on a sphere around the scene's centre,
+    looking at it (inside the enclosure, radius 0.9 around the icosphere;
+    radius 32 around the sphere field), elevation within +-35 degrees."""
+    az = rng.uniform(0.0, 2.0 * math.pi)
+    el = math.radians(rng.uniform(-35.0, 35.0))
+    dirv = (math.sin(az) * math.cos(el), math.sin(el), math.cos(az) * math.cos(el))
+    if kind == "ico":
+        c, rad, fov = (0.0, 0.6, 0.0), 0.9, 45.0
+    else:
+        c, rad, fov = (0.0, 0.0, 0.0), 32.0, 45.0
+    eye = tuple(c[i] + rad * dirv[i] for i in range(3))
+    return set_camera(params, eye, c, fov_deg=fov, width=width, height=height)

@@ -207,3 +207,31 @@ def test_scene_destroy_waits_for_server_jobs(ctx, resident):
         out = srv.wait(t, timeout_ms=60000)
     assert out is not None and out[1].status == 0
     assert np.array_equal(acc.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_wide_bvh_renders_the_same_image(resident):
+    """Path tracing over 4-wide nodes: the same closest hits except at
+    distance ties, so the images agree up to the rare tie pixels."""
+    ctx2 = Context(0)
+    hs, _ = resident["ico"]
+    p = params_of(hs, width=64, height=64, spp=8, max_depth=8)
+    ds = DeviceScene(ctx2, hs)
+    ctx2.set_tuning(bvh_width=2)
+    a2 = alone(ds, p)
+    ctx2.set_tuning(bvh_width=4)
+    a4 = alone(ds, p)
+    diff = np.abs(a2 - a4).max(axis=2) > 1e-4 * np.maximum(np.abs(a2).max(axis=2), 1.0)
+    assert diff.mean() < 0.01, diff.mean()
+
+
+def test_context_destroyed_without_rendering():
+    """A context that never rendered has no workspace (regression: its
+    destruction dereferenced the missing workspace)."""
+    import gc
+    c = Context(0)
+    del c
+    gc.collect()
+    c = Context(0)
+    c.set_tuning(bvh_width=4)
+    del c
+    gc.collect()

@@ -55,8 +55,17 @@ SCENES = ["intersection", "transforms", "spheres_raytrace", "cornell_raytrace", 
           "sphere"]
 
 
+@pytest.fixture(params=[2, 4], ids=["bvh2", "bvh4"])
+def width(request, ctx):
+    """Node width of the fast traversal mode (the exact mode is always binary)."""
+    old = ctx.get_tuning()["bvh_width"]
+    ctx.set_tuning(bvh_width=request.param)
+    yield request.param
+    ctx.set_tuning(bvh_width=old)
+
+
 @pytest.mark.parametrize("name", SCENES)
-def test_reference_scene_queries(ctx, oracle, ref_dir, name):
+def test_reference_scene_queries(ctx, width, oracle, ref_dir, name):
     path = os.path.join(ref_dir, name + ".muon")
     hs = HostScene.from_file(path, default_seed=1)
     ds = DeviceScene(ctx, hs)
@@ -84,7 +93,10 @@ def test_reference_scene_queries(ctx, oracle, ref_dir, name):
 
 def test_camera_rays_bit_exact(ctx, oracle, ref_dir):
     """Primary rays of the reference's normals scene hit the same primitive at
-    the same distance bit for bit."""
+    the same distance bit for bit (binary nodes visit children in the
+    reference's order, so even the scene's coplanar faces tie the same way)."""
+    old = ctx.get_tuning()["bvh_width"]
+    ctx.set_tuning(bvh_width=2)
     path = os.path.join(ref_dir, "cornell_normals.muon")
     hs = HostScene.from_file(path)
     ds = DeviceScene(ctx, hs)
@@ -103,7 +115,10 @@ def test_camera_rays_bit_exact(ctx, oracle, ref_dir):
     inv = np.float32(1.0) / np.sqrt(np.float32((dirs[:, 0] * dirs[:, 0] + dirs[:, 1] * dirs[:, 1]) + dirs[:, 2] * dirs[:, 2]))
     dirs = (dirs * inv[:, None]).astype(np.float32)
     org = np.tile(np.array(p.eye, np.float32), (dirs.shape[0], 1))
-    prim, t, _ = _query(ds, org, dirs)
+    try:
+        prim, t, _ = _query(ds, org, dirs)
+    finally:
+        ctx.set_tuning(bvh_width=old)
     pr, tr, _, _, _ = orc.intersect(org, dirs)
     assert np.array_equal(prim, pr)
     assert np.array_equal(t.view(np.uint32), tr.view(np.uint32))
@@ -118,7 +133,7 @@ def icosphere(ctx, oracle):
     return hs, ds, orc
 
 
-def test_icosphere_microbench_parity(icosphere):
+def test_icosphere_microbench_parity(icosphere, width):
     """Level-7 displaced icosphere (327,680 tris): the first 2^18 rays of the
     microbench ray sequence, closest-hit ids bit-exact vs the oracle."""
     hs, ds, orc = icosphere
@@ -130,7 +145,7 @@ def test_icosphere_microbench_parity(icosphere):
     assert exact > 0.9999
 
 
-def test_icosphere_full_microbench_invariants(icosphere):
+def test_icosphere_full_microbench_invariants(icosphere, width):
     """All 2^24 microbench rays: every hit is a valid primitive at positive
     distance; the oracle re-test of a random subset agrees."""
     hs, ds, orc = icosphere
