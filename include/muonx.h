@@ -181,6 +181,8 @@ typedef struct muonx_tuning {
   int32_t blocks_per_cu;
   int32_t path_budget_mb;
   int32_t bvh_width;  /* fast mode: 2 = binary nodes, 4 = 4-wide nodes collapsed from the same tree */
+  int32_t ray_fetch;  /* fast-mode queries: 0 = one ray per lane per grid step; 1..64 = a lane whose
+                         ray is done takes a new one once that many lanes of its wavefront are idle */
 } muonx_tuning;
 int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t);
 int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);

@@ -4,8 +4,9 @@
 //
 // Each iteration advances every queued path by one bounce:
 //   extend   closest hit of the queue's rays
-//   shade    emission, NEE shadow rays, MIS BRDF ray; a continuing path is
-//            appended to the next queue together with its state
+//   shade    emission and NEE shadow rays; then the MIS BRDF ray and the
+//            continuation: a continuing path is appended to the next queue
+//            together with its state
 //   shadow   any hit of the compacted shadow rays
 //   mis      closest hit of the MIS rays, light hits add the weighted emission
 //   retire   adds the NEE / MIS terms; continuing paths carry their radiance
@@ -56,6 +57,7 @@ struct EngCounters {
   uint32_t mis_count;
   uint32_t shadow_count;
   uint32_t plan_n, plan_base;
+  uint32_t claim[4];  // work claimed by the per-lane-fetch extend / shadow / mis kernels
   unsigned long long plan_cursor;
   unsigned long long cursor;          // next work item to generate
   unsigned long long total_items;     // admitted work (host-written)
@@ -84,7 +86,8 @@ struct EngArgs {
   uint32_t *nidx;     // its position in the next queue, or ~0 when it ends
   float4 *m_ray;      // MIS rays: (o.xyz, path index), (d.xyz, 0)
   float *m_fac, *m_add;
-  float4 *shift;      // origin of path i's shadow and MIS rays
+  float4 *shift;      // origin of path i's shadow and MIS rays (xyz), material id (w)
+  float4 *hnorm;      // shading normal at path i's hit
   // shadow rays, index k * capacity + i, and the queue of those to trace
   float4 *s_dir;      // (direction, tmax)
   float *s_contrib;

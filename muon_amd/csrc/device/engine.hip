@@ -6,44 +6,11 @@
 
 #include "engine.h"
 #include "hitrec.h"
+#include "wave.h"
 #include "shading.h"
 
 namespace muonx {
 
-
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
-  const uint64_t active = __ballot(1);
-  const uint64_t mask = __ballot(want);
-  const uint32_t lane = __lane_id();
-  const uint32_t leader = uint32_t(__ffsll((unsigned long long)active) - 1);
-  uint32_t base = 0;
-  if (lane == leader && mask) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-  base = uint32_t(__shfl(int(base), int(leader)));
-  return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-}
-
-// Reserves `cnt` consecutive entries per lane with one atomic per wavefront
-// (exclusive prefix sum over the lanes); all 64 lanes must be active.
-__device__ __forceinline__ uint32_t wave_append_n(uint32_t cnt, uint32_t *counter) {
-  const int lane = int(__lane_id());
-  uint32_t incl = cnt;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t v = uint32_t(__shfl_up(int(incl), off));
-    if (lane >= off) incl += v;
-  }
-  const uint32_t total = uint32_t(__shfl(int(incl), 63));
-  uint32_t base = 0;
-  if (lane == 63 && total) base = atomicAdd(counter, total);
-  base = uint32_t(__shfl(int(base), 63));
-  return base + incl - cnt;
-}
-
-__device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) {
-  unsigned long long a = n;
-  for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
-  if (__lane_id() == 0 && a) atomicAdd(ctr, a);
-}
 
 __device__ __forceinline__ unsigned long long to_fixed(float v) {
   if (!(v > 0.0f)) return 0ull;  // zero, negative or NaN
@@ -87,27 +54,26 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
 }
 
 // ---------------------------------------------------------------- shade
-__global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
+// PathTracer::Shade (muon/integration.cc:205-271) in two kernels, split where
+// the random draws of the light samples end, so each holds fewer registers:
+//   shade_light  hit record, emission, NEE shadow rays (ShadeDirectNEE)
+//   shade_bsdf   MIS BRDF ray (ShadeDirectImportanceSampling) and the
+//                continuation with Russian roulette (ShadeIndirect)
+__global__ void __launch_bounds__(kEngBlock) k_eng_shade_light(EngArgs a, int cur) {
   const uint32_t n = a.c->qcount[cur];
   const float4 *q = a.q_ray[cur];
-  float4 *qn = a.q_ray[cur ^ 1];
   const PathState &ps = a.st[cur];
-  const PathState &pn = a.st[cur ^ 1];
   const size_t P = a.capacity;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t i = base + threadIdx.x;
-    bool cont = false, want_mis = false;
-    uint32_t nshadow = 0, js = 0, pixel = 0, sample = 0, ctr = 0;
+    uint32_t nshadow = 0;
     uint64_t vmask = 0;  // valid shadow rays k < 64
-    int K = 0, depth = 0;
-    V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0}, T{0, 0, 0};
+    int K = 0;
     if (i < n) {
-      const float4 qo = q[2 * i], qd = q[2 * i + 1];
-      js = ps.job[i];
+      const uint32_t js = ps.job[i];
       const JobDev job = a.jobs[js];
-      const DevSceneView s = a.scenes[job.scene];
       const int32_t prim = a.hit_prim[i];
-      depth = ps.depth[i];
+      const int depth = ps.depth[i];
       K = job.K;
       for (int k = 0; k < K; ++k) {
         a.s_vis[k * P + i] = 0;
@@ -117,22 +83,28 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
       a.m_add[3 * i + 1] = 0.0f;
       a.m_add[3 * i + 2] = 0.0f;
       if (prim >= 0) {
+        const DevSceneView s = a.scenes[job.scene];
+        const float4 qo = q[2 * i], qd = q[2 * i + 1];
         const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
         const HitRecord h = hit_record(s, uint32_t(prim), ro, rd);
-        const Mat m = load_mat(s.materials + size_t(s.prim_mat[prim]) * kMatStride);
-        pixel = ps.pixel[i];
-        sample = ps.sample[i];
-        const uint32_t key = path_key(job.seed, job.key, pixel, sample);
-        ctr = ps.ctr[i];
-        T = V3{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
-        V3 rad{ps.rad[3 * i], ps.rad[3 * i + 1], ps.rad[3 * i + 2]};
-        shift = h.pos + kEps * h.normal;
-        a.shift[i] = make_float4(shift.x, shift.y, shift.z, 0.0f);
+        const int32_t mi = s.prim_mat[prim];
+        const V3 shift = h.pos + kEps * h.normal;
+        // the hit for shade_bsdf: shadow/MIS origin + material, normal
+        a.shift[i] = make_float4(shift.x, shift.y, shift.z, __int_as_float(mi));
+        a.hnorm[i] = make_float4(h.normal.x, h.normal.y, h.normal.z, 0.0f);
         if (depth >= job.min_depth) {
+          const Mat m = load_mat(s.materials + size_t(mi) * kMatStride);
+          const V3 T{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
+          V3 rad{ps.rad[3 * i], ps.rad[3 * i + 1], ps.rad[3 * i + 2]};
           // emission (muon/integration.cc:257-264)
           if (!((job.nee != MUONX_NEE_OFF && depth > 0) || dot(h.normal, -rd) < 0.0f)) rad = rad + T * m.emission;
+          ps.rad[3 * i] = rad.x;
+          ps.rad[3 * i + 1] = rad.y;
+          ps.rad[3 * i + 2] = rad.z;
           if (job.nee != MUONX_NEE_OFF) {
             // ShadeDirectNEE (muon/integration.cc:340-466): one shadow ray per light sample
+            const uint32_t key = path_key(job.seed, job.key, ps.pixel[i], ps.sample[i]);
+            uint32_t ctr = ps.ctr[i];
             const bool mis = job.nee == MUONX_NEE_MIS;
             int k = 0;
             for (uint32_t l = 0; l < s.num_lights; ++l) {
@@ -206,42 +178,79 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
                 }
               }
             }
-            if (mis) {
-              // ShadeDirectImportanceSampling (muon/integration.cc:468-501)
-              V3 dir, nT;
-              float bpdf;
-              if (sample_reflection(job.is_mode, m, rd, h.normal, T, key, ctr, dir, bpdf, nT)) {
-                const float lpdf = nee_pdf(s, shift, dir);
-                const V3 f = nT * power_heuristic(bpdf, lpdf);
-                a.m_fac[3 * i] = f.x;
-                a.m_fac[3 * i + 1] = f.y;
-                a.m_fac[3 * i + 2] = f.z;
-                want_mis = true;
-                mis_dir = dir;
-              }
-            }
+            ps.ctr[i] = ctr;
           }
         }
-        // ShadeIndirect (muon/integration.cc:294-328)
+      }
+    }
+    // the shadow rays to trace, compacted (zero contributions are skipped)
+    uint32_t si = wave_append_n(nshadow, &a.c->shadow_count);
+    for (uint64_t m = vmask; m; m &= m - 1) a.s_queue[si++] = uint32_t(size_t(__ffsll((unsigned long long)m) - 1) * P + i);
+    for (int k = 64; nshadow && k < K; ++k) {
+      const uint32_t sl = uint32_t(k * P + i);
+      if (a.s_valid[sl]) a.s_queue[si++] = sl;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) k_eng_shade_bsdf(EngArgs a, int cur) {
+  const uint32_t n = a.c->qcount[cur];
+  const float4 *q = a.q_ray[cur];
+  float4 *qn = a.q_ray[cur ^ 1];
+  const PathState &ps = a.st[cur];
+  const PathState &pn = a.st[cur ^ 1];
+  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+    const uint32_t i = base + threadIdx.x;
+    bool cont = false, want_mis = false;
+    uint32_t js = 0, pixel = 0, sample = 0, ctr = 0;
+    int depth = 0;
+    V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0}, T{0, 0, 0};
+    if (i < n && a.hit_prim[i] >= 0) {
+      js = ps.job[i];
+      const JobDev job = a.jobs[js];
+      const DevSceneView s = a.scenes[job.scene];
+      depth = ps.depth[i];
+      const float4 sh = a.shift[i], hn = a.hnorm[i];
+      shift = V3{sh.x, sh.y, sh.z};
+      const V3 normal{hn.x, hn.y, hn.z};
+      const Mat m = load_mat(s.materials + size_t(__float_as_int(sh.w)) * kMatStride);
+      const float4 qd = q[2 * i + 1];
+      const V3 rd{qd.x, qd.y, qd.z};
+      pixel = ps.pixel[i];
+      sample = ps.sample[i];
+      const uint32_t key = path_key(job.seed, job.key, pixel, sample);
+      ctr = ps.ctr[i];
+      T = V3{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
+      if (depth >= job.min_depth && job.nee == MUONX_NEE_MIS) {
+        // ShadeDirectImportanceSampling (muon/integration.cc:468-501)
         V3 dir, nT;
-        float pdf;
-        if (sample_reflection(job.is_mode, m, rd, h.normal, T, key, ctr, dir, pdf, nT)) {
-          bool alive = true;
-          if (job.rr) {
-            const float qc = gmin(gmax(gmax(nT.x, nT.y), nT.z), 1.0f);
-            if (qc < rng_uniform(key, ctr)) alive = false;
-            else nT = nT / qc;
-          }
-          // Trace() returns 0 past the depth cutoff (muon/integration.cc:29-33)
-          if (alive && depth + 1 <= job.cutoff && depth + 1 < kMaxBounces) {
-            cont = true;
-            next_dir = dir;
-            T = nT;
-          }
+        float bpdf;
+        if (sample_reflection(job.is_mode, m, rd, normal, T, key, ctr, dir, bpdf, nT)) {
+          const float lpdf = nee_pdf(s, shift, dir);
+          const V3 f = nT * power_heuristic(bpdf, lpdf);
+          a.m_fac[3 * i] = f.x;
+          a.m_fac[3 * i + 1] = f.y;
+          a.m_fac[3 * i + 2] = f.z;
+          want_mis = true;
+          mis_dir = dir;
         }
-        ps.rad[3 * i] = rad.x;
-        ps.rad[3 * i + 1] = rad.y;
-        ps.rad[3 * i + 2] = rad.z;
+      }
+      // ShadeIndirect (muon/integration.cc:294-328)
+      V3 dir, nT;
+      float pdf;
+      if (sample_reflection(job.is_mode, m, rd, normal, T, key, ctr, dir, pdf, nT)) {
+        bool alive = true;
+        if (job.rr) {
+          const float qc = gmin(gmax(gmax(nT.x, nT.y), nT.z), 1.0f);
+          if (qc < rng_uniform(key, ctr)) alive = false;
+          else nT = nT / qc;
+        }
+        // Trace() returns 0 past the depth cutoff (muon/integration.cc:29-33)
+        if (alive && depth + 1 <= job.cutoff && depth + 1 < kMaxBounces) {
+          cont = true;
+          next_dir = dir;
+          T = nT;
+        }
       }
     }
     // a continuing path moves to the next queue with its state; its radiance
@@ -264,13 +273,6 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade(EngArgs a, int cur) {
     if (want_mis) {
       a.m_ray[2 * mi] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(i));
       a.m_ray[2 * mi + 1] = make_float4(mis_dir.x, mis_dir.y, mis_dir.z, 0.0f);
-    }
-    // the shadow rays to trace, compacted (zero contributions are skipped)
-    uint32_t si = wave_append_n(nshadow, &a.c->shadow_count);
-    for (uint64_t m = vmask; m; m &= m - 1) a.s_queue[si++] = uint32_t(size_t(__ffsll((unsigned long long)m) - 1) * P + i);
-    for (int k = 64; nshadow && k < K; ++k) {
-      const uint32_t sl = uint32_t(k * P + i);
-      if (a.s_valid[sl]) a.s_queue[si++] = sl;
     }
   }
 }
@@ -330,6 +332,148 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, 
           a.m_add[3 * i + 2] = a.m_fac[3 * i + 2] * L[3];
         }
       }
+    }
+  }
+  wave_count(traced, &a.c->rays[1]);
+}
+
+// ---------------------------------------------------------------- per-lane fetch variants
+// The three traversal kernels with per-lane work claiming (WaveFetch): a
+// lane that finishes its ray takes the next one once `fetch_min` lanes of its
+// wavefront are idle. Results are those of the kernels above.
+template <int KIND, int WIDTH>
+__global__ void __launch_bounds__(kEngBlock) k_eng_extend_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
+  const uint32_t n = a.c->qcount[cur];
+  const float4 *q = a.q_ray[cur];
+  const PathState &ps = a.st[cur];
+  auto st = make_stack<KIND>(lds, kEngBlock, sa);
+  LaneTrace<WIDTH, false, false> lt;
+  DevSceneView sv;
+  TraceCounters c;
+  WaveFetch wf;
+  uint32_t i = 0, prim_rays = 0, sec_rays = 0;
+  bool have = false, drained = false;
+  while (true) {
+    const bool need = !have && !drained;
+    const uint64_t needm = __ballot(need);
+    if (needm && (__popcll(needm) >= fetch_min || __ballot(have) == 0)) {
+      const uint32_t k = wf.claim(needm, &a.c->claim[0]);
+      if (need) {
+        if (k < n) {
+          i = k;
+          const uint32_t js = ps.job[i];
+          const int depth = ps.depth[i];
+          // Integrator::Trace returns 0 past the depth cutoff (muon/integration.cc:29-33)
+          if (depth <= a.jobs[js].cutoff && depth < kMaxBounces) {
+            sv = a.scenes[a.jobs[js].scene];
+            const float4 qo = q[2 * i], qd = q[2 * i + 1];
+            lt.start(make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z}), __builtin_inff(), st);
+            have = true;
+            if (depth == 0) ++prim_rays;
+            else ++sec_rays;
+          } else {
+            a.hit_prim[i] = -1;
+          }
+        } else {
+          drained = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) break;
+    if (have && lt.round(sv, st, c)) {
+      a.hit_prim[i] = lt.hit;
+      have = false;
+    }
+  }
+  wave_count(prim_rays, &a.c->rays[0]);
+  wave_count(sec_rays, &a.c->rays[1]);
+}
+
+template <int KIND, int WIDTH>
+__global__ void __launch_bounds__(kEngBlock) k_eng_shadow_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
+  const uint32_t P = a.capacity;
+  const uint32_t n = a.c->shadow_count;
+  const PathState &ps = a.st[cur];
+  auto st = make_stack<KIND>(lds, kEngBlock, sa);
+  LaneTrace<WIDTH, true, false> lt;
+  DevSceneView sv;
+  TraceCounters c;
+  WaveFetch wf;
+  uint32_t sl = 0, traced = 0;
+  bool have = false, drained = false;
+  while (true) {
+    const bool need = !have && !drained;
+    const uint64_t needm = __ballot(need);
+    if (needm && (__popcll(needm) >= fetch_min || __ballot(have) == 0)) {
+      const uint32_t k = wf.claim(needm, &a.c->claim[1]);
+      if (need) {
+        if (k < n) {
+          sl = a.s_queue[k];
+          const uint32_t i = sl % P;
+          sv = a.scenes[a.jobs[ps.job[i]].scene];
+          const float4 so = a.shift[i], sd = a.s_dir[sl];
+          lt.start(make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}), sd.w, st);
+          have = true;
+          ++traced;
+        } else {
+          drained = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) break;
+    if (have && lt.round(sv, st, c)) {
+      a.s_vis[sl] = lt.hit < 0 ? 1 : 0;
+      have = false;
+    }
+  }
+  wave_count(traced, &a.c->rays[2]);
+}
+
+template <int KIND, int WIDTH>
+__global__ void __launch_bounds__(kEngBlock) k_eng_mis_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
+  const uint32_t n = a.c->mis_count;
+  const PathState &ps = a.st[cur];
+  auto st = make_stack<KIND>(lds, kEngBlock, sa);
+  LaneTrace<WIDTH, false, false> lt;
+  DevSceneView sv;
+  TraceCounters c;
+  WaveFetch wf;
+  uint32_t i = 0, traced = 0;
+  bool have = false, drained = false;
+  while (true) {
+    const bool need = !have && !drained;
+    const uint64_t needm = __ballot(need);
+    if (needm && (__popcll(needm) >= fetch_min || __ballot(have) == 0)) {
+      const uint32_t k = wf.claim(needm, &a.c->claim[2]);
+      if (need) {
+        if (k < n) {
+          const float4 mo = a.m_ray[2 * k], md = a.m_ray[2 * k + 1];
+          i = __float_as_uint(mo.w);
+          sv = a.scenes[a.jobs[ps.job[i]].scene];
+          lt.start(make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z}), __builtin_inff(), st);
+          have = true;
+          ++traced;
+        } else {
+          drained = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) break;
+    if (have && lt.round(sv, st, c)) {
+      const int32_t hit = lt.hit;
+      if (hit >= 0) {
+        const int32_t light = sv.prim_light[hit];
+        if (light >= 0) {
+          const float *L = sv.lights + size_t(light) * kLightStride;
+          a.m_add[3 * i] = a.m_fac[3 * i] * L[1];
+          a.m_add[3 * i + 1] = a.m_fac[3 * i + 1] * L[2];
+          a.m_add[3 * i + 2] = a.m_fac[3 * i + 2] * L[3];
+        }
+      }
+      have = false;
     }
   }
   wave_count(traced, &a.c->rays[1]);
@@ -395,6 +539,7 @@ __global__ void k_eng_plan(EngArgs a, int cur) {
   c->qcount[cur] = 0;
   c->mis_count = 0;
   c->shadow_count = 0;
+  c->claim[0] = c->claim[1] = c->claim[2] = 0;
   const uint32_t next = c->qcount[cur ^ 1];
   const unsigned long long avail = c->total_items > c->cursor ? c->total_items - c->cursor : 0ull;
   const uint32_t room = a.capacity - next;
@@ -486,6 +631,7 @@ size_t CarvePaths(uintptr_t base, size_t P, EngArgs &a) {
   a.m_fac = Carve<float>(p, 3 * P);
   a.m_add = Carve<float>(p, 3 * P);
   a.shift = Carve<float4>(p, P);
+  a.hnorm = Carve<float4>(p, P);
   return size_t(p - base);
 }
 
@@ -503,15 +649,25 @@ struct IterLaunch {
   const EngArgs &a;
   const TraverseCfg &cfg;
   int cur;
+  int fetch;  // per-lane fetch threshold, 0 = grid-stride kernels
   hipStream_t st;
   template <int KIND, bool WW, int WIDTH>
   void run() const {
     const StackArgs sa{cfg.spill, cfg.gcap};
     const dim3 grid(cfg.grid), block(kEngBlock);
-    hipLaunchKernelGGL((k_eng_extend<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
-    hipLaunchKernelGGL(k_eng_shade, grid, block, 0, st, a, cur);
-    hipLaunchKernelGGL((k_eng_shadow<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
-    hipLaunchKernelGGL((k_eng_mis<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
+    if (fetch > 0) {
+      hipLaunchKernelGGL((k_eng_extend_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
+      hipLaunchKernelGGL(k_eng_shade_light, grid, block, 0, st, a, cur);
+      hipLaunchKernelGGL(k_eng_shade_bsdf, grid, block, 0, st, a, cur);
+      hipLaunchKernelGGL((k_eng_shadow_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
+      hipLaunchKernelGGL((k_eng_mis_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
+    } else {
+      hipLaunchKernelGGL((k_eng_extend<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
+      hipLaunchKernelGGL(k_eng_shade_light, grid, block, 0, st, a, cur);
+      hipLaunchKernelGGL(k_eng_shade_bsdf, grid, block, 0, st, a, cur);
+      hipLaunchKernelGGL((k_eng_shadow<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
+      hipLaunchKernelGGL((k_eng_mis<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
+    }
     hipLaunchKernelGGL(k_eng_retire, grid, block, 0, st, a, cur);
     hipLaunchKernelGGL(k_eng_plan, dim3(1), dim3(64), 0, st, a, cur);
     hipLaunchKernelGGL(k_eng_fill, grid, block, 0, st, a, cur);
@@ -755,7 +911,7 @@ int Engine::SyncWindow() {
 }
 
 int Engine::LaunchIteration(int cur) {
-  DispatchTraverse(cfg_, IterLaunch{a_, cfg_, cur, stream_});
+  DispatchTraverse(cfg_, IterLaunch{a_, cfg_, cur, tuning_.ray_fetch, stream_});
   MX_HIP(hipGetLastError());
   return MUONX_OK;
 }

@@ -4,6 +4,7 @@
 #include "runtime.h"
 #include "trace.h"
 #include "traverse_launch.h"
+#include "wave.h"
 
 namespace muonx {
 
@@ -45,7 +46,86 @@ __global__ void __launch_bounds__(kQueryBlock) k_query(DevSceneView s, StackArgs
   }
 }
 
+// Fast mode with per-lane ray fetch: a lane whose ray is finished takes the
+// next unclaimed ray once at least `fetch_min` lanes of its wavefront are
+// idle, so long rays no longer hold idle lanes (claims: WaveFetch).
+
+template <int KIND, int WIDTH, bool ANY, bool COUNT>
+__global__ void __launch_bounds__(kQueryBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+    k_query_dyn(DevSceneView s, StackArgs sa, const float *__restrict__ ro, const float *__restrict__ rd,
+                const float *__restrict__ tmax, uint32_t n, int32_t *__restrict__ prim_out, float *__restrict__ t_out,
+                uint8_t *__restrict__ occ_out, unsigned long long *__restrict__ counters, uint32_t *next,
+                int fetch_min) {
+  __shared__ uint2 lds[StackOf<KIND>::kLds * kQueryBlock];
+  TraceCounters cnt{0, 0};
+  auto st = make_stack<KIND>(lds, kQueryBlock, sa);
+  LaneTrace<WIDTH, ANY, COUNT> lt;
+  uint32_t idx = 0;
+  WaveFetch wf;
+  bool have = false, drained = false;
+  while (true) {
+    const bool need = !have && !drained;
+    const uint64_t needm = __ballot(need);
+    if (needm && (__popcll(needm) >= fetch_min || __ballot(have) == 0)) {
+      const uint32_t k = wf.claim(needm, next);
+      if (need) {
+        if (k < n) {
+          idx = k;
+          const V3 o{ro[3 * k], ro[3 * k + 1], ro[3 * k + 2]};
+          const V3 d{rd[3 * k], rd[3 * k + 1], rd[3 * k + 2]};
+          lt.start(make_ray(o, d), tmax ? tmax[k] : __builtin_inff(), st);
+          have = true;
+        } else {
+          drained = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) break;
+    if (have && lt.round(s, st, cnt)) {
+      if (ANY) {
+        occ_out[idx] = lt.hit >= 0 ? 1 : 0;
+      } else {
+        prim_out[idx] = lt.hit >= 0 ? s.prim_id[lt.hit] : -1;
+        if (t_out) t_out[idx] = lt.hit >= 0 ? lt.best : 0.0f;
+      }
+      have = false;
+    }
+  }
+  if (COUNT) {
+    unsigned long long a = cnt.nodes, b = cnt.prims;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off);
+      b += __shfl_down(b, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&counters[0], a);
+      atomicAdd(&counters[1], b);
+    }
+  }
+}
+
 namespace {
+template <bool ANY, bool COUNT>
+struct QueryDynLaunch {
+  const TraverseCfg &cfg;
+  const DevSceneView &s;
+  const float *ro, *rd, *tmax;
+  uint32_t n;
+  int32_t *prim;
+  float *t;
+  uint8_t *occ;
+  unsigned long long *counters;
+  uint32_t *next;
+  int fetch_min;
+  hipStream_t stream;
+  template <int KIND, bool WW, int WIDTH>
+  void run() const {
+    const StackArgs sa{cfg.spill, cfg.gcap};
+    hipLaunchKernelGGL((k_query_dyn<KIND, WIDTH, ANY, COUNT>), dim3(cfg.grid), dim3(kQueryBlock), 0, stream, s, sa,
+                       ro, rd, tmax, n, prim, t, occ, counters, next, fetch_min);
+  }
+};
+
 template <bool ANY, bool COUNT, bool EXACT>
 struct QueryLaunch {
   const TraverseCfg &cfg;
@@ -77,6 +157,27 @@ int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, con
 #define MX_Q(A, C, E) DispatchTraverse(cfg, QueryLaunch<A, C, E>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, stream})
 #define MX_QE(A, C) DispatchExact(cfg, QueryLaunch<A, C, true>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, stream})
   const bool cnt = counters != nullptr;
+  const int fetch = ctx->tuning.ray_fetch;
+  if (!exact && fetch > 0 && n <= (1u << 31)) {  // the claim counter stays below 2^32 (n + chunk x waves)
+    // per-lane ray fetch; the claim counter is ordered like the spill region
+    MX_HIP(ctx->fetch_ctr.Reserve(sizeof(uint32_t)));
+    if (!ctx->spill_done) MX_HIP(hipEventCreateWithFlags(&ctx->spill_done, hipEventDisableTiming));
+    MX_HIP(hipStreamWaitEvent(stream, ctx->spill_done, 0));
+    MX_HIP(hipMemsetAsync(ctx->fetch_ctr.ptr, 0, sizeof(uint32_t), stream));
+    uint32_t *next = ctx->fetch_ctr.as<uint32_t>();
+#define MX_QD(A, C) DispatchTraverse(cfg, QueryDynLaunch<A, C>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, next, fetch, stream})
+    if (any) {
+      if (cnt) MX_QD(true, true);
+      else MX_QD(true, false);
+    } else {
+      if (cnt) MX_QD(false, true);
+      else MX_QD(false, false);
+    }
+#undef MX_QD
+    MX_HIP(hipGetLastError());
+    MX_HIP(hipEventRecord(ctx->spill_done, stream));
+    return MUONX_OK;
+  }
   if (exact) {
     // exact mode: full LDS stack, one node or leaf step per iteration
     if (any) MX_QE(true, false);

@@ -71,8 +71,9 @@ struct muonx_ctx {
   std::mutex mu;
   muonx::RenderWorkspace *ws = nullptr;
   int num_cus = 0;
-  muonx_tuning tuning{8, 1, 16, 2048, 4};  // best of the sweep on the level-7 icosphere
-  muonx::DevBuf spill;  // per-thread traversal stack spill region
+  muonx_tuning tuning{8, 1, 16, 2048, 4, 48};  // best of the sweep on the level-7 icosphere
+  muonx::DevBuf spill;      // per-thread traversal stack spill region
+  muonx::DevBuf fetch_ctr;  // ray claim counter of the per-lane fetch kernels
   // Orders kernels that share `spill` when they are issued on different streams.
   hipEvent_t spill_done = nullptr;
 };

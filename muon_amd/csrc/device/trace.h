@@ -179,6 +179,7 @@ struct LdsStack {
   int stride;
   int sp;
   __device__ __forceinline__ LdsStack(uint2 *b, int s) : base(b), stride(s), sp(0) {}
+  __device__ __forceinline__ void reset() { sp = 0; }
   __device__ __forceinline__ void push(uint2 v) {
     if (sp < CAP) {
       base[sp * stride] = v;
@@ -205,6 +206,7 @@ struct RingStack {
   int sp;
   __device__ __forceinline__ RingStack(uint2 *b, int s, uint2 *gb, int gs, int gc)
       : base(b), stride(s), g(gb), gstride(gs), gcap(gc), sp(0) {}
+  __device__ __forceinline__ void reset() { sp = 0; }
   __device__ __forceinline__ void push(uint2 v) {
     const int slot = sp & (S - 1);
     if (sp >= S) {
@@ -430,6 +432,39 @@ __device__ __forceinline__ int32_t trace_fast(const DevSceneView &s, const Trace
   if constexpr (WIDTH == 4) return trace_bvh4_s<ANY, COUNT, WW>(s, r, best, stack, cnt);
   else return trace_bvh_s<ANY, COUNT, false, WW>(s, r, best, stack, cnt);
 }
+
+// Resumable fast-mode traversal of one lane, for kernels that hand a lane a
+// new ray as soon as its current one is finished instead of waiting for the
+// slowest ray of the wavefront. round() walks internal nodes to the next leaf
+// and tests it (one while-while step); it returns true when the ray is done.
+template <int WIDTH, bool ANY, bool COUNT>
+struct LaneTrace {
+  TraceRay r;
+  float best;
+  int32_t hit;
+  uint32_t ref;
+  ObjCache oc;
+  template <class Stack>
+  __device__ __forceinline__ void start(const TraceRay &ray, float tmax, Stack &st) {
+    r = ray;
+    best = tmax;
+    hit = -1;
+    ref = 0;
+    oc.tid = -1;
+    st.reset();
+  }
+  template <class Stack>
+  __device__ __forceinline__ bool round(const DevSceneView &s, Stack &st, TraceCounters &cnt) {
+    while (!(ref & kLeafBit)) {
+      bool moved;
+      if constexpr (WIDTH == 4) moved = wnode_step<COUNT>(s, r, best, st, ref, cnt);
+      else moved = node_step<COUNT, false>(s, r, best, st, ref, cnt);
+      if (!moved && !pop_next(st, best, ref)) return true;
+    }
+    if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit, cnt)) return true;
+    return !pop_next(st, best, ref);
+  }
+};
 
 // Fixed LDS stack at stack[k * stride], k < STACK (STACK >= BVH depth).
 template <int STACK, bool ANY, bool COUNT, bool EXACT = false>

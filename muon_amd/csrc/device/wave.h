@@ -1,0 +1,80 @@
+// Wavefront-level helpers (64 lanes): queue appends with one atomic per
+// wavefront and counter reductions.
+#ifndef MUONX_DEVICE_WAVE_H_
+#define MUONX_DEVICE_WAVE_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace muonx {
+
+// Appends the lanes with `want` to a queue: returns each such lane's index
+// (lane order within the wavefront).
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
+  const uint64_t active = __ballot(1);
+  const uint64_t mask = __ballot(want);
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = uint32_t(__ffsll((unsigned long long)active) - 1);
+  uint32_t base = 0;
+  if (lane == leader && mask) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+  base = uint32_t(__shfl(int(base), int(leader)));
+  return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+// Reserves `cnt` consecutive entries per lane with one atomic per wavefront
+// (exclusive prefix sum over the lanes); all 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t cnt, uint32_t *counter) {
+  const int lane = int(__lane_id());
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = uint32_t(__shfl_up(int(incl), off));
+    if (lane >= off) incl += v;
+  }
+  const uint32_t total = uint32_t(__shfl(int(incl), 63));
+  uint32_t base = 0;
+  if (lane == 63 && total) base = atomicAdd(counter, total);
+  base = uint32_t(__shfl(int(base), 63));
+  return base + incl - cnt;
+}
+
+__device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) {
+  unsigned long long a = n;
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
+  if (__lane_id() == 0 && a) atomicAdd(ctr, a);
+}
+
+// Work claiming for kernels that give each lane a new item as soon as its
+// current one is finished: a wavefront takes chunks of kChunk consecutive
+// indices from a global counter (one atomic per chunk) and hands them to its
+// lanes in order. cbase / cend are the same in every lane.
+struct WaveFetch {
+  static constexpr uint32_t kChunk = 256;
+  uint32_t cbase = 0, cend = 0;
+  // Called by every lane of the wavefront with the ballot of `need`; returns
+  // the index claimed for this lane (meaningful where `need` is set).
+  __device__ __forceinline__ uint32_t claim(uint64_t needm, uint32_t *counter) {
+    const uint32_t lane = __lane_id();
+    const uint32_t cnt = uint32_t(__popcll(needm));
+    const uint32_t rank = uint32_t(__popcll(needm & ((1ull << lane) - 1ull)));
+    const uint32_t rem = cend - cbase;
+    uint32_t k;
+    if (rem >= cnt) {
+      k = cbase + rank;
+      cbase += cnt;
+    } else {
+      uint32_t nb = 0;
+      if (lane == 0) nb = atomicAdd(counter, kChunk);
+      nb = uint32_t(__shfl(int(nb), 0));
+      k = rank < rem ? cbase + rank : nb + (rank - rem);
+      cbase = nb + (cnt - rem);
+      cend = nb + kChunk;
+    }
+    return k;
+  }
+};
+
+}  // namespace muonx
+
+#endif

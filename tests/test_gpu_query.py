@@ -55,13 +55,14 @@ SCENES = ["intersection", "transforms", "spheres_raytrace", "cornell_raytrace", 
           "sphere"]
 
 
-@pytest.fixture(params=[2, 4], ids=["bvh2", "bvh4"])
+@pytest.fixture(params=[(2, 0), (4, 0), (2, 16), (4, 16)], ids=["bvh2", "bvh4", "bvh2-fetch", "bvh4-fetch"])
 def width(request, ctx):
-    """Node width of the fast traversal mode (the exact mode is always binary)."""
-    old = ctx.get_tuning()["bvh_width"]
-    ctx.set_tuning(bvh_width=request.param)
-    yield request.param
-    ctx.set_tuning(bvh_width=old)
+    """Node width of the fast traversal mode (the exact mode is always
+    binary), with and without per-lane ray fetch."""
+    old = ctx.get_tuning()
+    ctx.set_tuning(bvh_width=request.param[0], ray_fetch=request.param[1])
+    yield request.param[0]
+    ctx.set_tuning(bvh_width=old["bvh_width"], ray_fetch=old["ray_fetch"])
 
 
 @pytest.mark.parametrize("name", SCENES)
