@@ -17,7 +17,9 @@ CPU baseline: the CPU restatement of the reference (same BVH, same
 per-primitive object-space tests) on a bounded sample of the same rays,
 timed on this host.
 
-Also reported (outside the timed region):
+Also reported (outside the timed region), after muonx_autotune has chosen the
+launch configuration for this scene and render (the microbench above runs
+with the library's default configuration):
 * render: one full path-traced render of the same scene (512x512, 64 spp,
   depth 8, BRDF IS + NEE + MIS).
 * batched: 64 render jobs of uneven size (resolution uniform in
@@ -186,6 +188,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving workloads")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-load", type=float, default=0.7)
@@ -277,6 +280,18 @@ def main():
                 "device_nodes_per_ray": round(dev_nodes / args.rays, 2),
                 "device_prim_tests_per_ray": round(dev_prims / args.rays, 2)}
 
+    # the autotuner picks the launch configuration for this scene and the C1
+    # render job; the render, batched and serving measurements below use it
+    tune = None
+    if not args.no_autotune:
+        from muon_amd.scene import autotune
+        log("bench: autotune")
+        rep = autotune(ds, hs.params, 1 << 22)
+        tune = {"tuning": rep["tuning"], "candidates": rep["candidates"], "best_ms": round(rep["best_ms"], 4),
+                "default_ms": round(rep["previous_ms"], 4), "worst_ms": round(rep["worst_ms"], 4),
+                "sample_rays": rep["sample_rays"]}
+        log(f"bench: autotune {tune}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 8)
@@ -324,7 +339,8 @@ def main():
                                "(327,680 tris) in the Cornell enclosure",
                    "rays_per_step": args.rays, "prims": info["num_prims"], "bvh_depth": info["bvh_depth"],
                    "hit_fraction": round(hits / args.rays, 4), "wall_s": round(wall, 3)},
-        "roofline": roofline, "cpu_baseline": cpu, "render": render, "batched": batched, "serving": serving,
+        "roofline": roofline, "cpu_baseline": cpu, "autotune": tune, "render": render, "batched": batched,
+        "serving": serving,
     }
     if cpu:
         out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)

@@ -187,6 +187,23 @@ typedef struct muonx_tuning {
 int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t);
 int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);
 
+/* Autotuner (no counterpart in the reference): times candidate traversal
+ * configurations (bvh_width, stack_entries, while_while, blocks_per_cu,
+ * ray_fetch) on n_rays incoherent closest-hit rays over the scene's bounds
+ * (0: 2^22) and, when `job` is not NULL, the path-queue capacity
+ * (path_budget_mb) on that render (spp reduced to stay under 2^26 samples);
+ * installs the fastest in the scene's context. report may be NULL. */
+typedef struct muonx_autotune_report {
+  muonx_tuning tuning;     /* installed */
+  uint32_t candidates;     /* traversal configurations timed */
+  uint32_t sample_rays;
+  float best_ms, worst_ms; /* per launch of the sample rays */
+  float previous_ms;       /* the tuning in place before the call */
+  float render_ms;         /* the job at the chosen capacity, 0 without a job */
+} muonx_autotune_report;
+int muonx_autotune(muonx_scene *scene, const muonx_render_params *job, uint32_t n_rays,
+                   muonx_autotune_report *report);
+
 /* Uploads a scene; replaces the Structure::Init()'d acceleration structure
  * (muon/acceleration.h:26-53) with a device-resident BVH. Immutable after
  * creation and shareable by concurrent renders. */

@@ -59,6 +59,12 @@ class Tuning(C.Structure):
                 ("path_budget_mb", C.c_int32), ("bvh_width", C.c_int32), ("ray_fetch", C.c_int32)]
 
 
+class AutotuneReport(C.Structure):
+    _fields_ = [("tuning", Tuning), ("candidates", C.c_uint32), ("sample_rays", C.c_uint32),
+                ("best_ms", C.c_float), ("worst_ms", C.c_float), ("previous_ms", C.c_float),
+                ("render_ms", C.c_float)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("device_bytes", C.c_uint64), ("num_prims", C.c_uint32), ("num_nodes", C.c_uint32),
                 ("bvh_depth", C.c_uint32), ("num_lights", C.c_uint32), ("num_quad_lights", C.c_uint32)]
@@ -102,6 +108,7 @@ SIGNATURES = [
     ("muonx_ctx_destroy", None, [P]),
     ("muonx_ctx_set_tuning", C.c_int, [P, C.POINTER(Tuning)]),
     ("muonx_ctx_get_tuning", C.c_int, [P, C.POINTER(Tuning)]),
+    ("muonx_autotune", C.c_int, [P, C.POINTER(RenderParams), C.c_uint32, C.POINTER(AutotuneReport)]),
     ("muonx_scene_create", C.c_int, [P, C.POINTER(SceneDesc), C.POINTER(P)]),
     ("muonx_scene_destroy", None, [P]),
     ("muonx_scene_get_info", C.c_int, [P, C.POINTER(SceneInfo)]),

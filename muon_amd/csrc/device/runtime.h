@@ -86,6 +86,7 @@ struct muonx_scene {
   muonx::DevSceneView view{};
   uint32_t bvh_depth = 0;
   uint64_t device_bytes = 0;
+  float bounds[6] = {0, 0, 0, 0, 0, 0};  // world bounds: min xyz, max xyz
   std::vector<float> host_lights;     // 16 per light
   std::vector<float> host_materials;  // 16 per material
 };

@@ -355,6 +355,7 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   v.bvh_depth = depth;
   s->bvh_depth = depth;
   s->device_bytes = total;
+  std::memcpy(s->bounds, d->node_bounds, sizeof(s->bounds));  // the root's box
   s->host_lights = std::move(lights);
   s->host_materials = std::move(mats);
   s->id = RegisterScene();

@@ -100,6 +100,17 @@ class Context:
         check(lib.muonx_ctx_set_tuning(self.h, C.byref(_lib.Tuning(**cur))))
 
 
+def autotune(dev_scene, job=None, rays=0):
+    """muonx_autotune: times launch configurations on this scene (and the
+    render `job`, a RenderParams, if given), installs the fastest in the
+    scene's context and returns the report as a dict."""
+    r = _lib.AutotuneReport()
+    check(lib.muonx_autotune(dev_scene.h, C.byref(job) if job is not None else None, rays, C.byref(r)))
+    out = {k: getattr(r, k) for k, _ in r._fields_ if k != "tuning"}
+    out["tuning"] = {k: getattr(r.tuning, k) for k, _ in r.tuning._fields_}
+    return out
+
+
 class DeviceScene:
     """A scene resident in HBM (muonx_scene)."""
 

@@ -235,3 +235,35 @@ def test_context_destroyed_without_rendering():
     c.set_tuning(bvh_width=4)
     del c
     gc.collect()
+
+
+def test_autotune_installs_the_fastest_configuration(resident):
+    """muonx_autotune times the traversal candidates and the path-queue
+    capacity, installs the fastest, and queries keep their results."""
+    from muon_amd.scene import autotune
+    c = Context(0)
+    hs, _ = resident["ico"]
+    ds = DeviceScene(c, hs)
+    rng = np.random.default_rng(3)
+    lo, hi = scenes.icosphere_world_bounds(level=3)
+    o, d = scenes.microbench_rays(1 << 16, lo, hi, seed=4)
+    dev = torch.device("cuda:0")
+    to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    p0, t0 = ds.intersect_closest(to, td)
+    torch.cuda.synchronize()
+    p0, t0 = p0.cpu().numpy(), t0.cpu().numpy()
+    job = params_of(hs, width=48, height=48, spp=2, max_depth=4)
+    rep = autotune(ds, job, 1 << 18)
+    assert rep["candidates"] >= 24 and rep["sample_rays"] == 1 << 18
+    assert 0 < rep["best_ms"] <= rep["worst_ms"] and rep["best_ms"] <= rep["previous_ms"]
+    assert rep["render_ms"] > 0
+    assert c.get_tuning() == rep["tuning"]
+    p1, t1 = ds.intersect_closest(to, td)
+    torch.cuda.synchronize()
+    p1, t1 = p1.cpu().numpy(), t1.cpu().numpy()
+    same = p1 == p0
+    # node width and visit order can only change which of equal-distance hits wins
+    assert same.mean() > 0.999
+    assert np.allclose(t1[~same], t0[~same], rtol=1e-5)
+    img = alone(ds, job)
+    assert np.isfinite(img).all() and img.sum() > 0
