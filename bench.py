@@ -29,6 +29,9 @@ with the library's default configuration):
   fields), rendered as one batch on the persistent engine and as 64
   sequential plain muonx_render calls; images/s of both and whether every
   image is bit-identical between the two.
+  The same comparison for 128 small jobs (128^2, spp 4..16), where per-job
+  launch, tail and synchronisation costs dominate, is reported as
+  batched.small_jobs.
 * serving: the request loop under a Poisson stream of requests (resolution
   {256^2, 512^2, 1024^2}, spp 8..128, random scene and pose) at a fixed rate chosen
   as a load fraction of the batched throughput; images/s, p50/p99 latency
@@ -76,7 +79,7 @@ def make_job(pool, rng, res_choices, spp_lo, spp_hi, depth=8):
     return ds, p
 
 
-def bench_batched(ctx, pool, njobs, seed, log_fn):
+def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 1024), spp_range=(4, 64)):
     import numpy as np
     import torch
     from muon_amd.renderer import render_device
@@ -84,7 +87,7 @@ def bench_batched(ctx, pool, njobs, seed, log_fn):
     rng = np.random.default_rng(seed)
     # key 0 for every job: the plain path renders with key 0 too, so both
     # paths must produce the same images (the jobs differ by seed and pose)
-    jobs = [make_job(pool, rng, [128, 256, 512, 1024], 4, 64) + (0,) for _ in range(njobs)]
+    jobs = [make_job(pool, rng, list(res_choices), spp_range[0], spp_range[1]) + (0,) for _ in range(njobs)]
     samples = sum(p.width * p.height * p.spp for _, p, _ in jobs)
     film = sum(p.width * p.height * 24 + 256 for _, p, _ in jobs)
     cfg = engine_config(max_jobs=max(64, 2 * njobs), film_budget_bytes=film + (1 << 20), iterations_per_poll=4)
@@ -326,6 +329,10 @@ def main():
         pool = build_pool(ctx, log)
         batched, cap = bench_batched(ctx, pool, args.batch_jobs, 7, log)
         log(f"bench: batched {batched}")
+        small, _ = bench_batched(ctx, pool, 128, 8, log, res_choices=(128,), spp_range=(4, 16))
+        log(f"bench: batched small jobs {small}")
+        batched["small_jobs"] = {k: small[k] for k in ("jobs", "samples", "batched_s", "sequential_s", "images_per_s",
+                                                        "sequential_images_per_s", "speedup", "identical_to_plain")}
         serving = bench_serving(ctx, pool, args.serve_requests, cap, args.serve_load, 7, log)
         log(f"bench: serving {serving}")
         pool_desc = [f"{k}:{i['num_prims']}" for k, _, _, _, i in pool]

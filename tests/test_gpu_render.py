@@ -41,19 +41,25 @@ def _png(ref_dir, name):
     return np.asarray(Image.open(os.path.join(ref_dir, name)).convert("RGB"))
 
 
-@pytest.mark.parametrize("name", ["cornell_normals", "cornell_depth", "cornell_mesh", "partial_emission"])
+@pytest.mark.parametrize("name", ["cornell_normals", "cornell_depth", "cornell_mesh", "partial_emission",
+                                  "intersection", "transforms", "spheres_raytrace", "cornell_raytrace", "analytic"])
 def test_deterministic_film_matches_oracle_and_golden(ctx, oracle, ref_dir, name):
     path = os.path.join(ref_dir, name + ".muon")
     hs = HostScene.from_file(path, default_seed=3)
     acc, stats = gpu_render(ctx, hs)
     ref_acc, ref_rgb, ref_st, _ = oracle.load(path, seed=3).render(1)
-    assert np.abs(acc - ref_acc).max() <= 1e-5
-    # in practice the film carries the reference's bits
-    assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32))
+    assert (np.abs(acc - ref_acc) <= 1e-5 * np.maximum(np.abs(ref_acc), 1.0)).all()
+    if hs.params.integrator not in (3, 4):
+        # no transcendental calls: the film carries the reference's bits (the
+        # raytracer's powf and analyticdirect's acosf may differ by an ulp
+        # from the host library's)
+        assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32))
     rgb = Film(acc, hs.params.spp, hs.params.gamma).rgb8()
     gold = _png(ref_dir, name + ".png")
     assert int((rgb != gold).any(axis=2).sum()) == 0
     assert stats.primary_rays == ref_st[0]
+    if hs.params.integrator == 3:  # raytracer: reflection and shadow rays as the reference traces them
+        assert stats.secondary_rays == ref_st[1] and stats.shadow_rays == ref_st[6]
 
 
 def test_albedo_integrator(ctx, oracle, ref_dir):
