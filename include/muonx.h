@@ -190,9 +190,10 @@ int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);
 /* Autotuner (no counterpart in the reference): times candidate traversal
  * configurations (bvh_width, stack_entries, while_while, blocks_per_cu,
  * ray_fetch) on n_rays incoherent closest-hit rays over the scene's bounds
- * (0: 2^22) and, when `job` is not NULL, the path-queue capacity
- * (path_budget_mb) on that render (spp reduced to stay under 2^26 samples);
- * installs the fastest in the scene's context. report may be NULL. */
+ * (0: 2^22) and, when `job` is not NULL, the engine's ray_fetch,
+ * blocks_per_cu and path-queue capacity (path_budget_mb) on that render (spp
+ * reduced to stay under 2^26 samples); installs the fastest in the scene's
+ * context (with a job: the job's fastest). report may be NULL. */
 typedef struct muonx_autotune_report {
   muonx_tuning tuning;     /* installed */
   uint32_t candidates;     /* traversal configurations timed */

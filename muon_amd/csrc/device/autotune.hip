@@ -5,8 +5,9 @@
 // Traversal (BVH node width, stack form, while-while loop, blocks per CU,
 // per-lane ray fetch) is timed on incoherent closest-hit rays generated over
 // the scene's bounds: origins uniform in the box, directions uniform on the
-// sphere (counter-based draws). With a render job, the path-queue capacity
-// (tuning.path_budget_mb) is then timed on that job (at most 2^26 samples).
+// sphere (counter-based draws). With a render job, the engine's settings
+// (fetch threshold, blocks per CU, path-queue capacity) are then timed on
+// that job (at most 2^26 samples) and the job's best is installed.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -120,7 +121,6 @@ extern "C" int muonx_autotune(muonx_scene *scene, const muonx_render_params *job
     }
   }
   float render_ms = 0.0f;
-  int budget_best = ctx->tuning.path_budget_mb;
   if (rc == MUONX_OK && job) {
     // path-queue capacity on the job itself: the capacity matters when the
     // job has more samples than paths in flight, so a small proxy job would
@@ -135,27 +135,33 @@ extern "C" int muonx_autotune(muonx_scene *scene, const muonx_render_params *job
     }
     float best = 1e30f;
     const muonx_tuning tuned = ctx->tuning;
-    for (int mb : {1024, 2048, 4096, 8192}) {
-      {
-        std::lock_guard<std::mutex> lock(ctx->mu);
-        ctx->tuning.path_budget_mb = mb;
-      }
-      muonx_trace_stats st;
-      if ((rc = muonx_render(scene, &p, accum.as<float>(), &st, stream)) != MUONX_OK) break;
-      const auto t0 = std::chrono::steady_clock::now();
-      if ((rc = muonx_render(scene, &p, accum.as<float>(), &st, stream)) != MUONX_OK) break;
-      MX_HIP(hipStreamSynchronize(stream));
-      const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
-      if (ms < best) {
-        best = ms;
-        budget_best = mb;
-      }
-    }
+    muonx_tuning best_t = tuned;
+    // the engine's launch settings on the job: fetch threshold, grid, capacity
+    for (int fetch : {0, 48})
+      for (int bpc : {16, 24})
+        for (int mb : {2048, 4096, 8192}) {
+          muonx_tuning c = tuned;
+          c.ray_fetch = fetch;
+          c.blocks_per_cu = bpc;
+          c.path_budget_mb = mb;
+          {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->tuning = c;
+          }
+          muonx_trace_stats st;
+          if ((rc = muonx_render(scene, &p, accum.as<float>(), &st, stream)) != MUONX_OK) break;  // engine + graphs
+          const auto t0 = std::chrono::steady_clock::now();
+          if ((rc = muonx_render(scene, &p, accum.as<float>(), &st, stream)) != MUONX_OK) break;
+          MX_HIP(hipStreamSynchronize(stream));
+          const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+          if (ms < best) {
+            best = ms;
+            best_t = c;
+          }
+        }
     render_ms = best;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->tuning = tuned;
-    ctx->tuning.path_budget_mb = budget_best;
-    if (rc != MUONX_OK) ctx->tuning = base;
+    ctx->tuning = rc == MUONX_OK ? best_t : base;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
