@@ -183,6 +183,7 @@ typedef struct muonx_tuning {
   int32_t bvh_width;  /* fast mode: 2 = binary nodes, 4 = 4-wide nodes collapsed from the same tree */
   int32_t ray_fetch;  /* fast-mode queries: 0 = one ray per lane per grid step; 1..64 = a lane whose
                          ray is done takes a new one once that many lanes of its wavefront are idle */
+  int32_t node_bits;  /* 4-wide nodes: 32 = float child boxes (128 B), 8 = 8-bit quantized boxes (64 B) */
 } muonx_tuning;
 int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t);
 int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);
@@ -190,7 +191,7 @@ int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);
 /* Autotuner (no counterpart in the reference): times candidate traversal
  * configurations (bvh_width, stack_entries, while_while, blocks_per_cu,
  * ray_fetch) on n_rays incoherent closest-hit rays over the scene's bounds
- * (0: 2^22) and, when `job` is not NULL, the engine's ray_fetch,
+ * (0: 2^22) and, when `job` is not NULL, the engine's node_bits, ray_fetch,
  * blocks_per_cu and path-queue capacity (path_budget_mb) on that render (spp
  * reduced to stay under 2^26 samples); installs the fastest in the scene's
  * context (with a job: the job's fastest). report may be NULL. */

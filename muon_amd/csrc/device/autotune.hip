@@ -2,11 +2,12 @@
 // installs the fastest in the context's tuning. No counterpart in the
 // reference (its only knob is --parallelism).
 //
-// Traversal (BVH node width, stack form, while-while loop, blocks per CU,
-// per-lane ray fetch) is timed on incoherent closest-hit rays generated over
+// Traversal (BVH node width and box precision, stack form, while-while loop,
+// blocks per CU, per-lane ray fetch) is timed on incoherent closest-hit rays generated over
 // the scene's bounds: origins uniform in the box, directions uniform on the
 // sphere (counter-based draws). With a render job, the engine's settings
-// (fetch threshold, blocks per CU, path-queue capacity) are then timed on
+// (node precision, fetch threshold, blocks per CU, path-queue capacity) are
+// then timed on
 // that job (at most 2^26 samples) and the job's best is installed.
 #include <algorithm>
 #include <chrono>
@@ -86,13 +87,16 @@ extern "C" int muonx_autotune(muonx_scene *scene, const muonx_render_params *job
     MX_HIP(hipGetLastError());
     // candidate traversal configurations
     for (int width : {4, 2})
+      for (int bits : {32, 8})
       for (int stack : {8, 16, 0})
         for (int ww : {1, 0})
           for (int bpc : {8, 16, 24})
             for (int fetch : {0, 32, 48}) {
               if (fetch > 0 && ww == 0) continue;  // the fetch kernels always walk while-while
+              if (width == 2 && bits == 8) continue;  // quantized boxes exist for 4-wide nodes
               muonx_tuning c = base;
               c.bvh_width = width;
+              c.node_bits = bits;
               c.stack_entries = stack;
               c.while_while = ww;
               c.blocks_per_cu = bpc;
@@ -136,11 +140,15 @@ extern "C" int muonx_autotune(muonx_scene *scene, const muonx_render_params *job
     float best = 1e30f;
     const muonx_tuning tuned = ctx->tuning;
     muonx_tuning best_t = tuned;
-    // the engine's launch settings on the job: fetch threshold, grid, capacity
+    // the engine's launch settings on the job: node precision, fetch
+    // threshold, grid, capacity
+    for (int bits : {32, 8})
     for (int fetch : {0, 48})
       for (int bpc : {16, 24})
-        for (int mb : {2048, 4096, 8192}) {
+        for (int mb : {2048, 8192}) {
           muonx_tuning c = tuned;
+          if (c.bvh_width == 4) c.node_bits = bits;
+          else if (bits == 8) continue;
           c.ray_fetch = fetch;
           c.blocks_per_cu = bpc;
           c.path_budget_mb = mb;
@@ -176,7 +184,7 @@ extern "C" int muonx_autotune(muonx_scene *scene, const muonx_render_params *job
                                         [](const Candidate &x, const Candidate &y) { return x.ms < y.ms; });
     float base_ms = 0.0f;
     for (const Candidate &c : cands)
-      if (c.t.bvh_width == base.bvh_width && c.t.stack_entries == base.stack_entries &&
+      if (c.t.bvh_width == base.bvh_width && c.t.node_bits == base.node_bits && c.t.stack_entries == base.stack_entries &&
           c.t.while_while == base.while_while && c.t.blocks_per_cu == base.blocks_per_cu &&
           c.t.ray_fetch == base.ray_fetch)
         base_ms = c.ms;

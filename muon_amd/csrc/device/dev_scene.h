@@ -22,6 +22,15 @@
 //   w7 = (child count, 0, 0, 0)
 // The same leaves (primitive ranges) as the binary tree.
 //
+// Quantized wide BVH: the same nodes in 64 B = 4 x uint4, child planes as
+// 8-bit offsets from a per-node origin in steps of a per-axis power of two:
+//   q0 = (origin.xyz, biased exponents x | y << 8 | z << 16 | count << 24)
+//   q1 = (min.x[4], min.y[4], min.z[4], max.x[4]) one byte per child
+//   q2 = (max.y[4], max.z[4], 0, 0)
+//   q3 = child refs[4]
+// A decoded plane is origin + q * 2^e; every decoded box contains the float
+// box.
+//
 // Primitives (leaf order), 48 B = 3 x float4:
 //   tri:    (v0.xyz, tid)  (v1.xyz, 0)  (v2.xyz, 0)     object space
 //   sphere: (c.xyz, tid | kSphereBit)  (r, 0, 0, 0)  (0)
@@ -48,6 +57,7 @@ constexpr int kLightStride = 16;
 struct DevSceneView {
   const float4 *nodes;      // 4 per internal node
   const float4 *wnodes;     // 8 per wide node
+  const uint4 *qnodes;      // 4 per wide node, 8-bit quantized child boxes
   const float4 *prims;      // 3 per primitive, leaf order
   const float *xforms;      // 48 per transform
   const int32_t *prim_id;   // leaf order -> parse order

@@ -71,7 +71,7 @@ struct muonx_ctx {
   std::mutex mu;
   muonx::RenderWorkspace *ws = nullptr;
   int num_cus = 0;
-  muonx_tuning tuning{8, 1, 16, 2048, 4, 48};  // best of the sweep on the level-7 icosphere
+  muonx_tuning tuning{8, 1, 16, 2048, 4, 48, 8};  // best of the sweep on the level-7 icosphere
   muonx::DevBuf spill;      // per-thread traversal stack spill region
   muonx::DevBuf fetch_ctr;  // ray claim counter of the per-lane fetch kernels
   // Orders kernels that share `spill` when they are issued on different streams.
@@ -82,7 +82,7 @@ struct muonx_scene {
   muonx_ctx *ctx = nullptr;
   uint64_t id = 0;
   std::atomic<int> inflight{0};  // server jobs that use the scene
-  muonx::DevBuf nodes, wnodes, prims, xforms, prim_id, prim_mat, prim_light, prim_vn, prim_use_vn, materials, lights;
+  muonx::DevBuf nodes, wnodes, qnodes, prims, xforms, prim_id, prim_mat, prim_light, prim_vn, prim_use_vn, materials, lights;
   muonx::DevSceneView view{};
   uint32_t bvh_depth = 0;
   uint64_t device_bytes = 0;

@@ -2,6 +2,7 @@
 // binary SAH BVH in pre-order) into the device layout of dev_scene.h.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -109,6 +110,89 @@ std::vector<float4> CollapseWide(const muonx_scene_desc *d, uint32_t *max_stack)
     }
     q[6] = make_float4(BitsAsFloat(ref[0]), BitsAsFloat(ref[1]), BitsAsFloat(ref[2]), BitsAsFloat(ref[3]));
     q[7] = make_float4(BitsAsFloat(uint32_t(n)), 0.0f, 0.0f, 0.0f);
+  }
+  return out;
+}
+
+uint32_t FloatBits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof(u));
+  return u;
+}
+
+// The device's decode of a quantized plane: origin + q * 2^e, with the
+// multiplication exact and one rounding in the addition.
+float Dequant(float origin, uint32_t q, int e) { return origin + float(q) * std::ldexp(1.0f, e); }
+
+// Quantizes the 4-wide nodes (8 float4 each) to 64 B nodes (4 uint4 each):
+//   n0 = (origin.xyz, e.x + 127 | (e.y + 127) << 8 | (e.z + 127) << 16 | count << 24)
+//   n1 = (lo.x[4], lo.y[4], lo.z[4], hi.x[4]) as bytes (child k in byte k)
+//   n2 = (hi.y[4], hi.z[4], 0, 0)
+//   n3 = refs[4]
+// with child plane = origin + q * 2^e. Every decoded box contains the child's
+// float box (checked with the decode arithmetic itself), so a traversal over
+// the quantized nodes visits a superset of the boxes.
+std::vector<uint4> QuantizeWide(const std::vector<float4> &w) {
+  const size_t nn = w.size() / 8;
+  std::vector<uint4> out(nn * 4);
+  for (size_t i = 0; i < nn; ++i) {
+    const float4 *q = &w[i * 8];
+    const uint32_t n = FloatBits(q[7].x) <= 4 ? FloatBits(q[7].x) : 4;
+    float lo[3][4], hi[3][4];
+    for (int k = 0; k < 4; ++k) {
+      const float *l0 = &q[0].x, *h0 = &q[1].x, *l1 = &q[2].x, *h1 = &q[3].x, *l2 = &q[4].x, *h2 = &q[5].x;
+      lo[0][k] = l0[k];
+      hi[0][k] = h0[k];
+      lo[1][k] = l1[k];
+      hi[1][k] = h1[k];
+      lo[2][k] = l2[k];
+      hi[2][k] = h2[k];
+    }
+    float origin[3];
+    int ex[3];
+    uint32_t qlo[3][4] = {}, qhi[3][4] = {};
+    for (int a = 0; a < 3; ++a) {
+      float mn = lo[a][0], mx = hi[a][0];
+      for (uint32_t k = 1; k < n; ++k) {
+        mn = std::min(mn, lo[a][k]);
+        mx = std::max(mx, hi[a][k]);
+      }
+      origin[a] = mn;
+      const double extent = double(mx) - double(mn);
+      int e = -126;
+      if (extent > 0.0) {
+        int ee;
+        std::frexp(extent / 255.0, &ee);  // extent / 255 < 2^ee
+        e = std::max(-126, ee);
+      }
+      for (;; ++e) {
+        bool ok = e <= 127;
+        for (uint32_t k = 0; ok && k < n; ++k) {
+          const double sc = std::ldexp(1.0, e);
+          int64_t l = int64_t(std::floor((double(lo[a][k]) - double(mn)) / sc));
+          int64_t h = int64_t(std::ceil((double(hi[a][k]) - double(mn)) / sc));
+          l = std::max<int64_t>(0, std::min<int64_t>(255, l));
+          while (l > 0 && Dequant(mn, uint32_t(l), e) > lo[a][k]) --l;
+          h = std::max<int64_t>(0, h);
+          while (h <= 255 && Dequant(mn, uint32_t(h), e) < hi[a][k]) ++h;
+          if (h > 255) {
+            ok = false;
+            break;
+          }
+          qlo[a][k] = uint32_t(l);
+          qhi[a][k] = uint32_t(h);
+        }
+        if (ok || e >= 127) break;
+      }
+      ex[a] = e;
+    }
+    auto pack = [](const uint32_t v[4]) { return v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24); };
+    uint4 *o = &out[i * 4];
+    o[0] = make_uint4(FloatBits(origin[0]), FloatBits(origin[1]), FloatBits(origin[2]),
+                      uint32_t(ex[0] + 127) | (uint32_t(ex[1] + 127) << 8) | (uint32_t(ex[2] + 127) << 16) | (n << 24));
+    o[1] = make_uint4(pack(qlo[0]), pack(qlo[1]), pack(qlo[2]), pack(qhi[0]));
+    o[2] = make_uint4(pack(qhi[1]), pack(qhi[2]), 0u, 0u);
+    o[3] = make_uint4(FloatBits(q[6].x), FloatBits(q[6].y), FloatBits(q[6].z), FloatBits(q[6].w));
   }
   return out;
 }
@@ -320,7 +404,9 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   s->ctx = ctx;
   uint64_t total = 0;
   int rc = MUONX_OK;
+  const std::vector<uint4> qnodes = QuantizeWide(wnodes);
   if ((rc = Upload(s->nodes, nodes, total)) || (rc = Upload(s->wnodes, wnodes, total)) ||
+      (rc = Upload(s->qnodes, qnodes, total)) ||
       (rc = Upload(s->prims, prims, total)) ||
       (rc = Upload(s->xforms, xf, total)) || (rc = Upload(s->prim_id, pid, total)) ||
       (rc = Upload(s->prim_mat, pmat, total)) || (rc = Upload(s->prim_light, plight, total)) ||
@@ -337,6 +423,7 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   DevSceneView &v = s->view;
   v.nodes = s->nodes.as<float4>();
   v.wnodes = s->wnodes.as<float4>();
+  v.qnodes = s->qnodes.as<uint4>();
   v.num_wnodes = uint32_t(wnodes.size() / 8);
   v.wstack = wstack;
   v.prims = s->prims.as<float4>();

@@ -390,9 +390,74 @@ __device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay
   return true;
 }
 
+// Node formats of the fast traversal mode.
+constexpr int kNodesBinary = 2;  // binary child-boxes-in-parent nodes
+constexpr int kNodesWide = 4;    // 4-wide nodes, float child boxes
+constexpr int kNodesWideQ = 5;   // 4-wide nodes, 8-bit quantized child boxes
+
+__device__ __forceinline__ float dequant(float origin, uint32_t word, int k, float scale) {
+  return origin + float((word >> (8 * k)) & 255u) * scale;
+}
+
+// One quantized wide-node step (as wnode_step): decodes each child plane as
+// origin + q * 2^e (exact product, one rounding, as the host checked) and
+// applies the conservative slab test.
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
+                                           uint32_t &ref, TraceCounters &cnt) {
+  const uint4 *np = s.qnodes + 4 * size_t(ref);
+  const uint4 n0 = np[0], n1 = np[1], n2 = np[2], rf = np[3];
+  if (COUNT) cnt.nodes++;
+  const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
+  const float sx = __uint_as_float((n0.w & 255u) << 23), sy = __uint_as_float(((n0.w >> 8) & 255u) << 23);
+  const float sz = __uint_as_float(((n0.w >> 16) & 255u) << 23);
+  const uint32_t n = n0.w >> 24;
+  constexpr float kInf = __builtin_inff();
+  float t[4];
+  bool h[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    h[k] = box_hit(r, dequant(ox, n1.x, k, sx), dequant(ox, n1.w, k, sx), dequant(oy, n1.y, k, sy),
+                   dequant(oy, n2.x, k, sy), dequant(oz, n1.z, k, sz), dequant(oz, n2.y, k, sz), best, t[k]) &&
+           uint32_t(k) < n;
+  }
+  float k0 = h[0] ? t[0] : kInf, k1 = h[1] ? t[1] : kInf, k2 = h[2] ? t[2] : kInf, k3 = h[3] ? t[3] : kInf;
+  uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+  const int nhit = int(h[0]) + int(h[1]) + int(h[2]) + int(h[3]);
+  if (nhit == 0) return false;
+  auto cswap = [](float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
+    const bool sw = kb < ka;
+    const float tk = sw ? kb : ka;
+    const uint32_t tr = sw ? rb : ra;
+    kb = sw ? ka : kb;
+    rb = sw ? ra : rb;
+    ka = tk;
+    ra = tr;
+  };
+  cswap(k0, r0, k1, r1);
+  cswap(k2, r2, k3, r3);
+  cswap(k0, r0, k2, r2);
+  cswap(k1, r1, k3, r3);
+  cswap(k1, r1, k2, r2);
+  if (nhit > 3) stack.push(make_uint2(r3, __float_as_uint(k3)));
+  if (nhit > 2) stack.push(make_uint2(r2, __float_as_uint(k2)));
+  if (nhit > 1) stack.push(make_uint2(r1, __float_as_uint(k1)));
+  ref = r0;
+  return true;
+}
+
+// One internal-node step of the fast mode for node format NODES.
+template <int NODES, bool COUNT, class Stack>
+__device__ __forceinline__ bool fast_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
+                                          uint32_t &ref, TraceCounters &cnt) {
+  if constexpr (NODES == kNodesWideQ) return qnode_step<COUNT>(s, r, best, stack, ref, cnt);
+  else if constexpr (NODES == kNodesWide) return wnode_step<COUNT>(s, r, best, stack, ref, cnt);
+  else return node_step<COUNT, false>(s, r, best, stack, ref, cnt);
+}
+
 // Closest / any hit over the wide BVH (fast mode only: children are visited
 // nearest first, so ties may resolve differently from the reference's order).
-template <bool ANY, bool COUNT, bool WW, class Stack>
+template <int NODES, bool ANY, bool COUNT, bool WW, class Stack>
 __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const TraceRay &r, float &best, Stack &stack,
                                                 TraceCounters &cnt) {
   int32_t hit_prim = -1;
@@ -403,7 +468,7 @@ __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const Tra
     while (true) {
       bool alive = true;
       while (!(ref & kLeafBit)) {
-        if (!wnode_step<COUNT>(s, r, best, stack, ref, cnt) && !pop_next(stack, best, ref)) {
+        if (!fast_step<NODES, COUNT>(s, r, best, stack, ref, cnt) && !pop_next(stack, best, ref)) {
           alive = false;
           break;
         }
@@ -415,7 +480,7 @@ __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const Tra
   } else {
     while (true) {
       if (!(ref & kLeafBit)) {
-        if (wnode_step<COUNT>(s, r, best, stack, ref, cnt)) continue;
+        if (fast_step<NODES, COUNT>(s, r, best, stack, ref, cnt)) continue;
       } else {
         if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
       }
@@ -425,12 +490,12 @@ __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const Tra
   return hit_prim;
 }
 
-// Dispatch on the node width (2: binary child-boxes-in-parent, 4: wide).
+// Dispatch on the node format (kNodes*).
 template <int WIDTH, bool ANY, bool COUNT, bool WW, class Stack>
 __device__ __forceinline__ int32_t trace_fast(const DevSceneView &s, const TraceRay &r, float &best, Stack &stack,
                                               TraceCounters &cnt) {
-  if constexpr (WIDTH == 4) return trace_bvh4_s<ANY, COUNT, WW>(s, r, best, stack, cnt);
-  else return trace_bvh_s<ANY, COUNT, false, WW>(s, r, best, stack, cnt);
+  if constexpr (WIDTH == kNodesBinary) return trace_bvh_s<ANY, COUNT, false, WW>(s, r, best, stack, cnt);
+  else return trace_bvh4_s<WIDTH, ANY, COUNT, WW>(s, r, best, stack, cnt);
 }
 
 // Resumable fast-mode traversal of one lane, for kernels that hand a lane a
@@ -456,10 +521,7 @@ struct LaneTrace {
   template <class Stack>
   __device__ __forceinline__ bool round(const DevSceneView &s, Stack &st, TraceCounters &cnt) {
     while (!(ref & kLeafBit)) {
-      bool moved;
-      if constexpr (WIDTH == 4) moved = wnode_step<COUNT>(s, r, best, st, ref, cnt);
-      else moved = node_step<COUNT, false>(s, r, best, st, ref, cnt);
-      if (!moved && !pop_next(st, best, ref)) return true;
+      if (!fast_step<WIDTH, COUNT>(s, r, best, st, ref, cnt) && !pop_next(st, best, ref)) return true;
     }
     if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit, cnt)) return true;
     return !pop_next(st, best, ref);

@@ -15,7 +15,7 @@ namespace muonx {
 struct TraverseCfg {
   int kind;     // 24, 48, 96, 8 or 16
   bool ww;
-  int width;    // 2: binary nodes, 4: wide nodes
+  int width;    // node format: kNodesBinary, kNodesWide, kNodesWideQ
   int grid;     // persistent blocks
   int threads;  // grid * block
   uint2 *spill;
@@ -59,7 +59,7 @@ __device__ __forceinline__ typename StackOf<KIND>::T make_stack(uint2 *lds, int 
 inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_depth, uint32_t wstack,
                             uint64_t work_items, int block, DevBuf &spill, TraverseCfg &cfg, bool want_exact = false) {
   const int cus = num_cus > 0 ? num_cus : 256;
-  cfg.width = (!want_exact && t.bvh_width == 4) ? 4 : 2;
+  cfg.width = want_exact || t.bvh_width != 4 ? kNodesBinary : (t.node_bits == 8 ? kNodesWideQ : kNodesWide);
   const int bpc = std::max(1, t.blocks_per_cu);
   cfg.grid = int(std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * bpc, (work_items + block - 1) / block)));
   cfg.threads = cfg.grid * block;
@@ -67,7 +67,7 @@ inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_dep
   cfg.spill = nullptr;
   cfg.gcap = 0;
   // binary: entries never exceed depth - 1; wide: the collapse's bound
-  const int need = int(cfg.width == 4 ? std::max<uint32_t>(wstack, 1) : bvh_depth);
+  const int need = int(cfg.width != kNodesBinary ? std::max<uint32_t>(wstack, 1) : bvh_depth);
   const bool ring = !want_exact && (t.stack_entries == 8 || t.stack_entries == 16);
   if (ring && need <= t.stack_entries) {
     cfg.kind = need <= 8 ? 8 : 16;  // a ring that never spills
@@ -123,8 +123,9 @@ inline void DispatchKind(const TraverseCfg &cfg, F &&f) {
 }
 template <class F>
 inline void DispatchTraverse(const TraverseCfg &cfg, F &&f) {
-  if (cfg.width == 4) DispatchKind<4>(cfg, f);
-  else DispatchKind<2>(cfg, f);
+  if (cfg.width == kNodesWideQ) DispatchKind<kNodesWideQ>(cfg, f);
+  else if (cfg.width == kNodesWide) DispatchKind<kNodesWide>(cfg, f);
+  else DispatchKind<kNodesBinary>(cfg, f);
 }
 // Exact mode: binary nodes, full LDS stack, if-if loop.
 template <class F>

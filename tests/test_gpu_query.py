@@ -55,14 +55,17 @@ SCENES = ["intersection", "transforms", "spheres_raytrace", "cornell_raytrace", 
           "sphere"]
 
 
-@pytest.fixture(params=[(2, 0), (4, 0), (2, 16), (4, 16)], ids=["bvh2", "bvh4", "bvh2-fetch", "bvh4-fetch"])
+@pytest.fixture(params=[(2, 0, 32), (4, 0, 32), (4, 0, 8), (2, 16, 32), (4, 16, 32), (4, 48, 8)],
+                ids=["bvh2", "bvh4", "bvh4q", "bvh2-fetch", "bvh4-fetch", "bvh4q-fetch"])
 def width(request, ctx):
-    """Node width of the fast traversal mode (the exact mode is always
-    binary), with and without per-lane ray fetch."""
+    """Node format of the fast traversal mode (the exact mode is always
+    binary): binary, 4-wide float boxes, 4-wide 8-bit boxes; with and
+    without per-lane ray fetch."""
     old = ctx.get_tuning()
-    ctx.set_tuning(bvh_width=request.param[0], ray_fetch=request.param[1])
-    yield request.param[0]
-    ctx.set_tuning(bvh_width=old["bvh_width"], ray_fetch=old["ray_fetch"])
+    w, f, b = request.param
+    ctx.set_tuning(bvh_width=w, ray_fetch=f, node_bits=b)
+    yield w
+    ctx.set_tuning(bvh_width=old["bvh_width"], ray_fetch=old["ray_fetch"], node_bits=old["node_bits"])
 
 
 @pytest.mark.parametrize("name", SCENES)
