@@ -21,7 +21,8 @@ Also reported (outside the timed region), after muonx_autotune has chosen the
 launch configuration for this scene and render (the microbench above runs
 with the library's default configuration):
 * render: one full path-traced render of the same scene (512x512, 64 spp,
-  depth 8, BRDF IS + NEE + MIS).
+  depth 8, BRDF IS + NEE + MIS); render.sphere_field: the 10,000-sphere field
+  with 8 emissive spheres (1024x1024, 32 spp, depth 8, BRDF sampling, NEE off).
 * batched: 64 render jobs of uneven size (resolution uniform in
   {128^2, 256^2, 512^2, 1024^2}, spp uniform in [4, 64], depth 8, seeded
   camera poses)
@@ -190,6 +191,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-sphere-field", action="store_true", help="render only the icosphere scene")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving workloads")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -309,20 +311,31 @@ def main():
                "sample": f"first {done} of the same microbench rays, closest hit, {threads} threads, {secs:.1f} s"}
         log(f"bench: cpu baseline {cpu['value']} Mrays/s")
 
-    render = None
-    if not args.no_render:
-        log("bench: path-traced render 512x512 64spp depth 8")
-        accum, rst = render_device(ds, hs.params)  # warm the workspace
+    def timed_render(scene, params):
+        accum, rst = render_device(scene, params)  # warm the workspace
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        accum, rst = render_device(ds, hs.params)
+        accum, rst = render_device(scene, params)
         torch.cuda.synchronize()
         rs = time.perf_counter() - t0
         rays_total = rst.primary_rays + rst.secondary_rays + rst.shadow_rays
-        render = {"seconds": round(rs, 4), "images_per_s": round(1.0 / rs, 3),
-                  "mrays_per_s": round(rays_total / rs / 1e6, 1), "primary_rays": rst.primary_rays,
-                  "secondary_rays": rst.secondary_rays, "shadow_rays": rst.shadow_rays,
-                  "mean_radiance": float(accum.mean().item()) / hs.params.spp}
+        return {"seconds": round(rs, 4), "images_per_s": round(1.0 / rs, 3),
+                "mrays_per_s": round(rays_total / rs / 1e6, 1), "primary_rays": rst.primary_rays,
+                "secondary_rays": rst.secondary_rays, "shadow_rays": rst.shadow_rays,
+                "mean_radiance": float(accum.mean().item()) / params.spp}
+
+    render = None
+    if not args.no_render:
+        log("bench: path-traced render 512x512 64spp depth 8")
+        render = timed_render(ds, hs.params)
+        log(f"bench: render {render}")
+    if not args.no_render and not args.no_sphere_field:
+        log("bench: sphere field 1024x1024 32spp depth 8 (10,008 spheres, BRDF sampling, NEE off)")
+        hs2 = HostScene.from_text(scenes.sphere_field_text())
+        ds2 = DeviceScene(ctx, hs2)
+        render["sphere_field"] = timed_render(ds2, hs2.params)
+        log(f"bench: sphere field {render['sphere_field']}")
+        del ds2
 
     batched = serving = None
     if not args.no_batch:

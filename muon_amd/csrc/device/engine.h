@@ -8,8 +8,9 @@
 //            continuation: a continuing path is appended to the next queue
 //            together with its state
 //   shadow   any hit of the compacted shadow rays
-//   mis      closest hit of the MIS rays, light hits add the weighted emission
-//   retire   adds the NEE / MIS terms; continuing paths carry their radiance
+//   mis      closest hit of the MIS rays
+//   retire   adds the NEE terms and, for MIS rays that hit a light, the
+//            weighted emission; continuing paths carry their radiance
 //            into the next queue, finished ones add it to their job's film
 //   plan     decides how many new samples fit behind the continuing paths
 //   fill     appends camera samples of admitted jobs, in work order
@@ -35,6 +36,8 @@
 namespace muonx {
 
 constexpr int kEngBlock = 128;
+constexpr int kShadeBlock = 256;  // shade kernels: one queue-append atomic per block
+constexpr int kShadeWaves = kShadeBlock / 64;
 constexpr int kMaxBounces = 1024;  // hard cap for max_depth = -1 without Russian roulette
 constexpr double kFilmScale = 16777216.0;  // fixed-point units per radiance unit
 
@@ -52,12 +55,20 @@ struct JobDev {
 
 // Device-side scalars of the engine (one struct so that the host can update
 // the admission window with a single copy).
+// A counter on a 256-byte line of its own: atomics on one L2 line are
+// executed one after another, so counters that many wavefronts update are
+// kept apart.
+struct alignas(256) HotCounter {
+  uint32_t v;
+  uint32_t pad[63];
+};
+
 struct EngCounters {
-  uint32_t qcount[2];  // path queues
-  uint32_t mis_count;
-  uint32_t shadow_count;
+  HotCounter qcount[2];  // path queues
+  HotCounter mis_count;
+  HotCounter shadow_count;
+  HotCounter claim[3];   // work claimed by the per-lane-fetch extend / shadow / mis kernels
   uint32_t plan_n, plan_base;
-  uint32_t claim[4];  // work claimed by the per-lane-fetch extend / shadow / mis kernels
   unsigned long long plan_cursor;
   unsigned long long cursor;          // next work item to generate
   unsigned long long total_items;     // admitted work (host-written)
@@ -85,7 +96,8 @@ struct EngArgs {
   int32_t *hit_prim;  // closest hit of path i of the current queue
   uint32_t *nidx;     // its position in the next queue, or ~0 when it ends
   float4 *m_ray;      // MIS rays: (o.xyz, path index), (d.xyz, 0)
-  float *m_fac, *m_add;
+  float *m_fac;       // MIS weight times the BRDF-sampled throughput
+  int32_t *m_hit;     // closest hit of the MIS ray, -1 for none
   float4 *shift;      // origin of path i's shadow and MIS rays (xyz), material id (w)
   float4 *hnorm;      // shading normal at path i's hit
   // shadow rays, index k * capacity + i, and the queue of those to trace

@@ -22,7 +22,7 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 template <int KIND, bool WW, int WIDTH>
 __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
-  const uint32_t n = a.c->qcount[cur];
+  const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
   const PathState &ps = a.st[cur];
   uint32_t prim_rays = 0, sec_rays = 0;
@@ -59,12 +59,13 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
 //   shade_light  hit record, emission, NEE shadow rays (ShadeDirectNEE)
 //   shade_bsdf   MIS BRDF ray (ShadeDirectImportanceSampling) and the
 //                continuation with Russian roulette (ShadeIndirect)
-__global__ void __launch_bounds__(kEngBlock) k_eng_shade_light(EngArgs a, int cur) {
-  const uint32_t n = a.c->qcount[cur];
+__global__ void __launch_bounds__(kShadeBlock) k_eng_shade_light(EngArgs a, int cur) {
+  __shared__ uint32_t wsum[kShadeWaves + 1];
+  const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
   const PathState &ps = a.st[cur];
   const size_t P = a.capacity;
-  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
     uint32_t nshadow = 0;
     uint64_t vmask = 0;  // valid shadow rays k < 64
@@ -79,9 +80,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade_light(EngArgs a, int cu
         a.s_vis[k * P + i] = 0;
         a.s_valid[k * P + i] = 0;
       }
-      a.m_add[3 * i] = 0.0f;
-      a.m_add[3 * i + 1] = 0.0f;
-      a.m_add[3 * i + 2] = 0.0f;
+      a.m_hit[i] = -1;
       if (prim >= 0) {
         const DevSceneView s = a.scenes[job.scene];
         const float4 qo = q[2 * i], qd = q[2 * i + 1];
@@ -184,7 +183,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade_light(EngArgs a, int cu
       }
     }
     // the shadow rays to trace, compacted (zero contributions are skipped)
-    uint32_t si = wave_append_n(nshadow, &a.c->shadow_count);
+    uint32_t si = block_append_n<kShadeWaves>(nshadow, &a.c->shadow_count.v, wsum);
     for (uint64_t m = vmask; m; m &= m - 1) a.s_queue[si++] = uint32_t(size_t(__ffsll((unsigned long long)m) - 1) * P + i);
     for (int k = 64; nshadow && k < K; ++k) {
       const uint32_t sl = uint32_t(k * P + i);
@@ -193,13 +192,15 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shade_light(EngArgs a, int cu
   }
 }
 
-__global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) k_eng_shade_bsdf(EngArgs a, int cur) {
-  const uint32_t n = a.c->qcount[cur];
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+    k_eng_shade_bsdf(EngArgs a, int cur) {
+  __shared__ uint32_t wsum[kShadeWaves + 1];
+  const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
   float4 *qn = a.q_ray[cur ^ 1];
   const PathState &ps = a.st[cur];
   const PathState &pn = a.st[cur ^ 1];
-  for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
     bool cont = false, want_mis = false;
     uint32_t js = 0, pixel = 0, sample = 0, ctr = 0;
@@ -255,7 +256,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
     }
     // a continuing path moves to the next queue with its state; its radiance
     // follows in retire, after the NEE / MIS terms are added
-    const uint32_t ni = wave_append(cont, &a.c->qcount[cur ^ 1]);
+    const uint32_t ni = block_append<kShadeWaves>(cont, &a.c->qcount[cur ^ 1].v, wsum);
     if (i < n) a.nidx[i] = cont ? ni : 0xffffffffu;
     if (cont) {
       qn[2 * ni] = make_float4(shift.x, shift.y, shift.z, 0.0f);
@@ -269,7 +270,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
       pn.thr[3 * ni + 1] = T.y;
       pn.thr[3 * ni + 2] = T.z;
     }
-    const uint32_t mi = wave_append(want_mis, &a.c->mis_count);
+    const uint32_t mi = block_append<kShadeWaves>(want_mis, &a.c->mis_count.v, wsum);
     if (want_mis) {
       a.m_ray[2 * mi] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(i));
       a.m_ray[2 * mi + 1] = make_float4(mis_dir.x, mis_dir.y, mis_dir.z, 0.0f);
@@ -282,7 +283,7 @@ template <int KIND, bool WW, int WIDTH>
 __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t P = a.capacity;
-  const uint32_t n = a.c->shadow_count;
+  const uint32_t n = a.c->shadow_count.v;
   const PathState &ps = a.st[cur];
   uint32_t traced = 0;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
@@ -308,7 +309,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs s
 template <int KIND, bool WW, int WIDTH>
 __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
-  const uint32_t n = a.c->mis_count;
+  const uint32_t n = a.c->mis_count.v;
   const PathState &ps = a.st[cur];
   uint32_t traced = 0;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
@@ -321,17 +322,8 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, 
       float best = __builtin_inff();
       TraceCounters c;
       auto st = make_stack<KIND>(lds, kEngBlock, sa);
-      const int32_t hit = trace_fast<WIDTH, false, false, WW>(s, r, best, st, c);
+      a.m_hit[i] = trace_fast<WIDTH, false, false, WW>(s, r, best, st, c);
       ++traced;
-      if (hit >= 0) {
-        const int32_t light = s.prim_light[hit];
-        if (light >= 0) {
-          const float *L = s.lights + size_t(light) * kLightStride;
-          a.m_add[3 * i] = a.m_fac[3 * i] * L[1];
-          a.m_add[3 * i + 1] = a.m_fac[3 * i + 1] * L[2];
-          a.m_add[3 * i + 2] = a.m_fac[3 * i + 2] * L[3];
-        }
-      }
     }
   }
   wave_count(traced, &a.c->rays[1]);
@@ -342,9 +334,9 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, 
 // lane that finishes its ray takes the next one once `fetch_min` lanes of its
 // wavefront are idle. Results are those of the kernels above.
 template <int KIND, int WIDTH>
-__global__ void __launch_bounds__(kEngBlock) k_eng_extend_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
+__global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) k_eng_extend_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
-  const uint32_t n = a.c->qcount[cur];
+  const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
   const PathState &ps = a.st[cur];
   auto st = make_stack<KIND>(lds, kEngBlock, sa);
@@ -358,7 +350,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend_dyn(EngArgs a, StackAr
     const bool need = !have && !drained;
     const uint64_t needm = __ballot(need);
     if (needm && (__popcll(needm) >= fetch_min || __ballot(have) == 0)) {
-      const uint32_t k = wf.claim(needm, &a.c->claim[0]);
+      const uint32_t k = wf.claim(needm, &a.c->claim[0].v);
       if (need) {
         if (k < n) {
           i = k;
@@ -391,10 +383,10 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend_dyn(EngArgs a, StackAr
 }
 
 template <int KIND, int WIDTH>
-__global__ void __launch_bounds__(kEngBlock) k_eng_shadow_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
+__global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) k_eng_shadow_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t P = a.capacity;
-  const uint32_t n = a.c->shadow_count;
+  const uint32_t n = a.c->shadow_count.v;
   const PathState &ps = a.st[cur];
   auto st = make_stack<KIND>(lds, kEngBlock, sa);
   LaneTrace<WIDTH, true, false> lt;
@@ -407,7 +399,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shadow_dyn(EngArgs a, StackAr
     const bool need = !have && !drained;
     const uint64_t needm = __ballot(need);
     if (needm && (__popcll(needm) >= fetch_min || __ballot(have) == 0)) {
-      const uint32_t k = wf.claim(needm, &a.c->claim[1]);
+      const uint32_t k = wf.claim(needm, &a.c->claim[1].v);
       if (need) {
         if (k < n) {
           sl = a.s_queue[k];
@@ -432,9 +424,9 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shadow_dyn(EngArgs a, StackAr
 }
 
 template <int KIND, int WIDTH>
-__global__ void __launch_bounds__(kEngBlock) k_eng_mis_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
+__global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) k_eng_mis_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
-  const uint32_t n = a.c->mis_count;
+  const uint32_t n = a.c->mis_count.v;
   const PathState &ps = a.st[cur];
   auto st = make_stack<KIND>(lds, kEngBlock, sa);
   LaneTrace<WIDTH, false, false> lt;
@@ -447,7 +439,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis_dyn(EngArgs a, StackArgs 
     const bool need = !have && !drained;
     const uint64_t needm = __ballot(need);
     if (needm && (__popcll(needm) >= fetch_min || __ballot(have) == 0)) {
-      const uint32_t k = wf.claim(needm, &a.c->claim[2]);
+      const uint32_t k = wf.claim(needm, &a.c->claim[2].v);
       if (need) {
         if (k < n) {
           const float4 mo = a.m_ray[2 * k], md = a.m_ray[2 * k + 1];
@@ -463,16 +455,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis_dyn(EngArgs a, StackArgs 
     }
     if (__ballot(have) == 0) break;
     if (have && lt.round(sv, st, c)) {
-      const int32_t hit = lt.hit;
-      if (hit >= 0) {
-        const int32_t light = sv.prim_light[hit];
-        if (light >= 0) {
-          const float *L = sv.lights + size_t(light) * kLightStride;
-          a.m_add[3 * i] = a.m_fac[3 * i] * L[1];
-          a.m_add[3 * i + 1] = a.m_fac[3 * i + 1] * L[2];
-          a.m_add[3 * i + 2] = a.m_fac[3 * i + 2] * L[3];
-        }
-      }
+      a.m_hit[i] = lt.hit;  // retire adds the weighted emission when it is a light
       have = false;
     }
   }
@@ -482,12 +465,14 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis_dyn(EngArgs a, StackArgs 
 // ---------------------------------------------------------------- retire
 // Adds each shaded path's NEE and MIS terms in a fixed order. A continuing
 // path's radiance moves to its place in the next queue; a finished path adds
-// it to the job's film. Done counters take one atomic per job per wavefront.
+// it to the job's film and counts towards the job's done counter.
 __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
-  const uint32_t n = a.c->qcount[cur];
+  const uint32_t n = a.c->qcount[cur].v;
   const PathState &ps = a.st[cur];
   const PathState &pn = a.st[cur ^ 1];
   const size_t P = a.capacity;
+  uint32_t carry_job = 0xffffffffu;  // same in every lane
+  uint64_t carry_n = 0;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t i = base + threadIdx.x;
     bool fin = false;
@@ -502,7 +487,19 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
           const size_t sl = k * P + i;
           if (a.s_vis[sl]) direct = direct + V3{a.s_contrib[3 * sl], a.s_contrib[3 * sl + 1], a.s_contrib[3 * sl + 2]};
         }
-        direct = direct + V3{a.m_add[3 * i], a.m_add[3 * i + 1], a.m_add[3 * i + 2]};
+        // the MIS BRDF ray: a light hit adds next_T * color * w
+        // (muon/integration.cc:490-499), zero otherwise
+        V3 madd{0.0f, 0.0f, 0.0f};
+        const int32_t mh = a.m_hit[i];
+        if (mh >= 0) {
+          const DevSceneView &s = a.scenes[a.jobs[js].scene];
+          const int32_t light = s.prim_light[mh];
+          if (light >= 0) {
+            const float *L = s.lights + size_t(light) * kLightStride;
+            madd = V3{a.m_fac[3 * i] * L[1], a.m_fac[3 * i + 1] * L[2], a.m_fac[3 * i + 2] * L[3]};
+          }
+        }
+        direct = direct + madd;
         rad = rad + direct;
       }
       const uint32_t ni = a.nidx[i];
@@ -519,15 +516,26 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
         fin = true;
       }
     }
+    // done counters: the wavefront carries (job, count) across iterations
+    // and adds it when the job changes, so one job costs about one atomic per
+    // wavefront instead of one per iteration (the host reads them after the
+    // kernel)
     uint64_t pending = __ballot(fin);
     while (pending) {
       const int leader = __ffsll((unsigned long long)pending) - 1;
       const uint32_t jl = uint32_t(__shfl(int(js), leader));
       const uint64_t same = __ballot(fin && js == jl);
-      if (int(__lane_id()) == leader) atomicAdd(a.jobs[jl].done, (unsigned long long)__popcll(same));
+      if (jl == carry_job) {
+        carry_n += uint64_t(__popcll(same));
+      } else {
+        if (carry_n && __lane_id() == 0) atomicAdd(a.jobs[carry_job].done, (unsigned long long)carry_n);
+        carry_job = jl;
+        carry_n = uint64_t(__popcll(same));
+      }
       pending &= ~same;
     }
   }
+  if (carry_n && __lane_id() == 0) atomicAdd(a.jobs[carry_job].done, (unsigned long long)carry_n);
 }
 
 // ---------------------------------------------------------------- plan / fill
@@ -536,11 +544,11 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
 __global__ void k_eng_plan(EngArgs a, int cur) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   EngCounters *c = a.c;
-  c->qcount[cur] = 0;
-  c->mis_count = 0;
-  c->shadow_count = 0;
-  c->claim[0] = c->claim[1] = c->claim[2] = 0;
-  const uint32_t next = c->qcount[cur ^ 1];
+  c->qcount[cur].v = 0;
+  c->mis_count.v = 0;
+  c->shadow_count.v = 0;
+  c->claim[0].v = c->claim[1].v = c->claim[2].v = 0;
+  const uint32_t next = c->qcount[cur ^ 1].v;
   const unsigned long long avail = c->total_items > c->cursor ? c->total_items - c->cursor : 0ull;
   const uint32_t room = a.capacity - next;
   const uint32_t nfill = uint32_t(avail < room ? avail : room);
@@ -548,7 +556,7 @@ __global__ void k_eng_plan(EngArgs a, int cur) {
   c->plan_base = next;
   c->plan_cursor = c->cursor;
   c->cursor += nfill;
-  c->qcount[cur ^ 1] = next + nfill;
+  c->qcount[cur ^ 1].v = next + nfill;
 }
 
 __global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
@@ -629,7 +637,7 @@ size_t CarvePaths(uintptr_t base, size_t P, EngArgs &a) {
   a.hit_prim = Carve<int32_t>(p, P);
   a.nidx = Carve<uint32_t>(p, P);
   a.m_fac = Carve<float>(p, 3 * P);
-  a.m_add = Carve<float>(p, 3 * P);
+  a.m_hit = Carve<int32_t>(p, P);
   a.shift = Carve<float4>(p, P);
   a.hnorm = Carve<float4>(p, P);
   return size_t(p - base);
@@ -655,16 +663,17 @@ struct IterLaunch {
   void run() const {
     const StackArgs sa{cfg.spill, cfg.gcap};
     const dim3 grid(cfg.grid), block(kEngBlock);
+    const dim3 sgrid(std::max(1, cfg.grid * kEngBlock / kShadeBlock)), sblock(kShadeBlock);
     if (fetch > 0) {
       hipLaunchKernelGGL((k_eng_extend_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
-      hipLaunchKernelGGL(k_eng_shade_light, grid, block, 0, st, a, cur);
-      hipLaunchKernelGGL(k_eng_shade_bsdf, grid, block, 0, st, a, cur);
+      hipLaunchKernelGGL(k_eng_shade_light, sgrid, sblock, 0, st, a, cur);
+      hipLaunchKernelGGL(k_eng_shade_bsdf, sgrid, sblock, 0, st, a, cur);
       hipLaunchKernelGGL((k_eng_shadow_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
       hipLaunchKernelGGL((k_eng_mis_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
     } else {
       hipLaunchKernelGGL((k_eng_extend<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
-      hipLaunchKernelGGL(k_eng_shade_light, grid, block, 0, st, a, cur);
-      hipLaunchKernelGGL(k_eng_shade_bsdf, grid, block, 0, st, a, cur);
+      hipLaunchKernelGGL(k_eng_shade_light, sgrid, sblock, 0, st, a, cur);
+      hipLaunchKernelGGL(k_eng_shade_bsdf, sgrid, sblock, 0, st, a, cur);
       hipLaunchKernelGGL((k_eng_shadow<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
       hipLaunchKernelGGL((k_eng_mis<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
     }

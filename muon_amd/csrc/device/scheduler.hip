@@ -64,8 +64,8 @@ uint64_t FilmBytes(const muonx_render_params &p) {
 }
 
 // Device bytes of one path in flight (two queue-ordered states 88, rays and
-// per-path terms 160, 34 per shadow ray), as Engine::Create lays them out.
-uint64_t PathSlotBytes(int kmax) { return 88 + 160 + 34 * uint64_t(kmax); }
+// per-path terms 152, 34 per shadow ray), as Engine::Create lays them out.
+uint64_t PathSlotBytes(int kmax) { return 88 + 152 + 34 * uint64_t(kmax); }
 
 }  // namespace
 

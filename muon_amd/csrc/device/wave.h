@@ -39,6 +39,55 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t cnt, uint32_t *counte
   return base + incl - cnt;
 }
 
+// Block-wide versions: NW wavefronts (blockDim.x = 64 * NW) reserve their
+// entries with one atomic per block, in block order. Every thread of the
+// block must call them (they synchronise the block); `sh` is NW + 1 words of
+// shared memory.
+template <int NW>
+__device__ __forceinline__ uint32_t block_append_n(uint32_t cnt, uint32_t *counter, uint32_t *sh) {
+  const int lane = int(__lane_id());
+  const int w = int(threadIdx.x) >> 6;
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = uint32_t(__shfl_up(int(incl), off));
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) sh[w] = incl;  // the wavefront's total
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) tot += sh[k];
+    sh[NW] = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = sh[NW];
+  for (int k = 0; k < w; ++k) base += sh[k];
+  __syncthreads();  // sh is reused by the next call
+  return base + incl - cnt;
+}
+
+template <int NW>
+__device__ __forceinline__ uint32_t block_append(bool want, uint32_t *counter, uint32_t *sh) {
+  const uint64_t m = __ballot(want);
+  const int lane = int(__lane_id());
+  const int w = int(threadIdx.x) >> 6;
+  if (lane == 0) sh[w] = uint32_t(__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) tot += sh[k];
+    sh[NW] = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = sh[NW];
+  for (int k = 0; k < w; ++k) base += sh[k];
+  __syncthreads();
+  return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+}
+
 __device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) {
   unsigned long long a = n;
   for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
