@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
   LaneTrace<WIDTH, false, false> lt;
   DevSceneView sv;
   TraceCounters c;
-  WaveFetch wf;
+  WaveFetch wf(n);
   uint32_t i = 0, prim_rays = 0, sec_rays = 0;
   bool have = false, drained = false;
   while (true) {
@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
   LaneTrace<WIDTH, true, false> lt;
   DevSceneView sv;
   TraceCounters c;
-  WaveFetch wf;
+  WaveFetch wf(n);
   uint32_t sl = 0, traced = 0;
   bool have = false, drained = false;
   while (true) {
@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
   LaneTrace<WIDTH, false, false> lt;
   DevSceneView sv;
   TraceCounters c;
-  WaveFetch wf;
+  WaveFetch wf(n);
   uint32_t i = 0, traced = 0;
   bool have = false, drained = false;
   while (true) {

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kQueryBlock) __attribute__((amdgpu_waves_per_e
   auto st = make_stack<KIND>(lds, kQueryBlock, sa);
   LaneTrace<WIDTH, ANY, COUNT> lt;
   uint32_t idx = 0;
-  WaveFetch wf;
+  WaveFetch wf(n);
   bool have = false, drained = false;
   while (true) {
     const bool need = !have && !drained;

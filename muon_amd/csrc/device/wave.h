@@ -95,14 +95,29 @@ __device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) 
 }
 
 // Work claiming for kernels that give each lane a new item as soon as its
-// current one is finished: a wavefront takes chunks of kChunk consecutive
-// indices from a global counter (one atomic per chunk) and hands them to its
-// lanes in order. cbase / cend are the same in every lane.
+// current one is finished. Items are handed out in chunks of `chunk`
+// consecutive indices (64..kMaxChunk, about four chunks per wavefront of the
+// grid): wavefront w's first chunk is [w * chunk, (w + 1) * chunk) with no
+// atomic, later chunks come from a global counter (one atomic per chunk,
+// offset past the first chunks of all wavefronts). A wavefront whose first
+// chunk lies past the item count exits without touching the counter, so a
+// launch over few or no items costs no contended atomics. cbase / cend /
+// chunk are the same in every lane.
 struct WaveFetch {
-  static constexpr uint32_t kChunk = 256;
-  uint32_t cbase = 0, cend = 0;
+  static constexpr uint32_t kMaxChunk = 256;
+  uint32_t chunk, first, cbase, cend;
+  __device__ __forceinline__ explicit WaveFetch(uint32_t items) {
+    const uint32_t wpb = (blockDim.x + 63u) >> 6;
+    const uint32_t waves = gridDim.x * wpb;
+    const uint32_t c = (items / (4u * waves)) & ~63u;
+    chunk = c < 64u ? 64u : (c > kMaxChunk ? kMaxChunk : c);
+    first = waves * chunk;
+    cbase = (blockIdx.x * wpb + (threadIdx.x >> 6)) * chunk;
+    cend = cbase + chunk;
+  }
   // Called by every lane of the wavefront with the ballot of `need`; returns
-  // the index claimed for this lane (meaningful where `need` is set).
+  // the index claimed for this lane (meaningful where `need` is set; an
+  // index >= the item count means none is left).
   __device__ __forceinline__ uint32_t claim(uint64_t needm, uint32_t *counter) {
     const uint32_t lane = __lane_id();
     const uint32_t cnt = uint32_t(__popcll(needm));
@@ -114,11 +129,11 @@ struct WaveFetch {
       cbase += cnt;
     } else {
       uint32_t nb = 0;
-      if (lane == 0) nb = atomicAdd(counter, kChunk);
+      if (lane == 0) nb = first + atomicAdd(counter, chunk);
       nb = uint32_t(__shfl(int(nb), 0));
       k = rank < rem ? cbase + rank : nb + (rank - rem);
       cbase = nb + (cnt - rem);
-      cend = nb + kChunk;
+      cend = nb + chunk;
     }
     return k;
   }
