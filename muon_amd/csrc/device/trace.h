@@ -345,6 +345,13 @@ __device__ __forceinline__ int32_t trace_bvh_s(const DevSceneView &s, const Trac
   return hit_prim;
 }
 
+// Makes a loaded value count as used at this point. A node's child
+// references are needed only when a child box is hit; without this the
+// compiler sinks their load below the box test, which costs a second
+// dependent fetch of the node on every step. The empty asm moves nothing and
+// reads no memory; the reference load then issues with the box loads.
+__device__ __forceinline__ void use_now(uint4 &v) { asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
+
 // One wide-node step: tests the (up to 4) child boxes with the conservative
 // slab test, moves `ref` to the nearest hit child and pushes the other hit
 // children farthest first (so they pop nearest first). Returns false when no
@@ -354,7 +361,7 @@ __device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay
                                            uint32_t &ref, TraceCounters &cnt) {
   const float4 *np = s.wnodes + 8 * size_t(ref);
   const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-  const uint4 rf = *reinterpret_cast<const uint4 *>(np + 6);
+  uint4 rf = *reinterpret_cast<const uint4 *>(np + 6);
   const uint32_t n = __float_as_uint(np[7].x);
   if (COUNT) cnt.nodes++;
   constexpr float kInf = __builtin_inff();
@@ -364,6 +371,7 @@ __device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay
   const bool h2 = box_hit(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best, t2) && n > 2;
   const bool h3 = box_hit(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best, t3) && n > 3;
   // (distance, ref) keys; misses sort last
+  use_now(rf);
   float k0 = h0 ? t0 : kInf, k1 = h1 ? t1 : kInf, k2 = h2 ? t2 : kInf, k3 = h3 ? t3 : kInf;
   uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
   const int nhit = int(h0) + int(h1) + int(h2) + int(h3);
@@ -406,7 +414,8 @@ template <bool COUNT, class Stack>
 __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
                                            uint32_t &ref, TraceCounters &cnt) {
   const uint4 *np = s.qnodes + 4 * size_t(ref);
-  const uint4 n0 = np[0], n1 = np[1], n2 = np[2], rf = np[3];
+  const uint4 n0 = np[0], n1 = np[1], n2 = np[2];
+  uint4 rf = np[3];
   if (COUNT) cnt.nodes++;
   const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
   const float sx = __uint_as_float((n0.w & 255u) << 23), sy = __uint_as_float(((n0.w >> 8) & 255u) << 23);
@@ -421,6 +430,7 @@ __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay
                    dequant(oy, n2.x, k, sy), dequant(oz, n1.z, k, sz), dequant(oz, n2.y, k, sz), best, t[k]) &&
            uint32_t(k) < n;
   }
+  use_now(rf);
   float k0 = h[0] ? t[0] : kInf, k1 = h[1] ? t[1] : kInf, k2 = h[2] ? t[2] : kInf, k3 = h[3] ? t[3] : kInf;
   uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
   const int nhit = int(h[0]) + int(h[1]) + int(h[2]) + int(h[3]);
