@@ -403,8 +403,10 @@ constexpr int kNodesBinary = 2;  // binary child-boxes-in-parent nodes
 constexpr int kNodesWide = 4;    // 4-wide nodes, float child boxes
 constexpr int kNodesWideQ = 5;   // 4-wide nodes, 8-bit quantized child boxes
 
+// origin + q * 2^e: the product is exact (8-bit q, power-of-two scale), so
+// the fused form rounds once exactly like the host's add, in one instruction.
 __device__ __forceinline__ float dequant(float origin, uint32_t word, int k, float scale) {
-  return origin + float((word >> (8 * k)) & 255u) * scale;
+  return __builtin_fmaf(float((word >> (8 * k)) & 255u), scale, origin);
 }
 
 // One quantized wide-node step (as wnode_step): decodes each child plane as
