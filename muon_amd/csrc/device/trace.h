@@ -49,9 +49,15 @@ __device__ __forceinline__ TraceRay make_ray(V3 o, V3 d) {
   return r;
 }
 
-// Conservative slab test; returns the widened entry distance in `lo`.
+// The largest float below b, for b > 0: l < b exactly when l <= below(b).
+__device__ __forceinline__ float below(float b) { return __uint_as_float(__float_as_uint(b) - 1u); }
+
+// Conservative slab test against `best_m` = below(best); returns the widened
+// entry distance in `lo`. Accepts when the widened interval [l, h] is not
+// empty and starts before best (l >= 0, so h >= 0; a box touching the origin
+// from behind, h == 0, is visited and finds nothing).
 __device__ __forceinline__ bool box_hit(const TraceRay &r, float mnx, float mxx, float mny, float mxy, float mnz,
-                                        float mxz, float best, float &lo) {
+                                        float mxz, float best_m, float &lo) {
   float ax = (mnx - r.o.x) * r.inv.x, bx = (mxx - r.o.x) * r.inv.x;
   float ay = (mny - r.o.y) * r.inv.y, by = (mxy - r.o.y) * r.inv.y;
   float az = (mnz - r.o.z) * r.inv.z, bz = (mxz - r.o.z) * r.inv.z;
@@ -60,7 +66,7 @@ __device__ __forceinline__ bool box_hit(const TraceRay &r, float mnx, float mxx,
   l = l * kWidenLo;
   h = h * kWidenHi;
   lo = l;
-  return l <= h && h > 0.0f && l < best;
+  return l <= fminf(h, best_m);
 }
 
 // The reference's slab test (muon/bounds.cc:88-125) with its divisions and
@@ -186,6 +192,23 @@ struct LdsStack {
       ++sp;
     }
   }
+  // Pushes c, b, a (the last m of them, m <= 3) in that order: a ends on top.
+  // Three unconditional writes, c then b then a: an entry that is not pushed
+  // is written to a's slot first and overwritten by a.
+  __device__ __forceinline__ void push_last(int m, uint2 a, uint2 b, uint2 c) {
+    if (m <= 0) return;
+    if (sp + m > CAP) {
+      if (m >= 3) push(c);
+      if (m >= 2) push(b);
+      push(a);
+      return;
+    }
+    const int pa = sp + m - 1, pb = m >= 2 ? sp + m - 2 : pa, pc = m >= 3 ? sp : pa;
+    base[pc * stride] = c;
+    base[pb * stride] = b;
+    base[pa * stride] = a;
+    sp += m;
+  }
   __device__ __forceinline__ bool empty() const { return sp == 0; }
   __device__ __forceinline__ uint2 pop() {
     --sp;
@@ -215,6 +238,24 @@ struct RingStack {
     }
     base[slot * stride] = v;
     ++sp;
+  }
+  // Pushes c, b, a (the last m of them, m <= 3) in that order: a ends on top.
+  // Without a spill (the common case) the entries go straight to their ring
+  // slots (all below S) with three unconditional writes, c then b then a: an
+  // entry that is not pushed is written to a's slot first and overwritten.
+  __device__ __forceinline__ void push_last(int m, uint2 a, uint2 b, uint2 c) {
+    if (m <= 0) return;
+    if (sp + m > S) {
+      if (m >= 3) push(c);
+      if (m >= 2) push(b);
+      push(a);
+      return;
+    }
+    const int pa = sp + m - 1, pb = m >= 2 ? sp + m - 2 : pa, pc = m >= 3 ? sp : pa;
+    base[pc * stride] = c;
+    base[pb * stride] = b;
+    base[pa * stride] = a;
+    sp += m;
   }
   __device__ __forceinline__ bool empty() const { return sp == 0; }
   __device__ __forceinline__ uint2 pop() {
@@ -250,8 +291,9 @@ __device__ __forceinline__ bool node_step(const DevSceneView &s, const TraceRay 
     h0 = box_hit_exact(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
     h1 = box_hit_exact(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
   } else {
-    h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, best, lo0);
-    h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, best, lo1);
+    const float bm = below(best);
+    h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, bm, lo0);
+    h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, bm, lo1);
   }
   const uint32_t axis = (uint32_t(n3.z) >> 29) & 3u;
   const bool swap = (r.dneg >> axis) & 1u;  // near child is children[1]
@@ -366,10 +408,11 @@ __device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay
   if (COUNT) cnt.nodes++;
   constexpr float kInf = __builtin_inff();
   float t0, t1, t2, t3;
-  const bool h0 = box_hit(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, best, t0);
-  const bool h1 = box_hit(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, best, t1) && n > 1;
-  const bool h2 = box_hit(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best, t2) && n > 2;
-  const bool h3 = box_hit(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best, t3) && n > 3;
+  const float bm = below(best);
+  const bool h0 = box_hit(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, bm, t0);
+  const bool h1 = box_hit(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, bm, t1) && n > 1;
+  const bool h2 = box_hit(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, bm, t2) && n > 2;
+  const bool h3 = box_hit(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, bm, t3) && n > 3;
   // (distance, ref) keys; misses sort last
   use_now(rf);
   float k0 = h0 ? t0 : kInf, k1 = h1 ? t1 : kInf, k2 = h2 ? t2 : kInf, k3 = h3 ? t3 : kInf;
@@ -391,9 +434,8 @@ __device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay
   cswap(k0, r0, k2, r2);
   cswap(k1, r1, k3, r3);
   cswap(k1, r1, k2, r2);
-  if (nhit > 3) stack.push(make_uint2(r3, __float_as_uint(k3)));
-  if (nhit > 2) stack.push(make_uint2(r2, __float_as_uint(k2)));
-  if (nhit > 1) stack.push(make_uint2(r1, __float_as_uint(k1)));
+  stack.push_last(nhit - 1, make_uint2(r1, __float_as_uint(k1)), make_uint2(r2, __float_as_uint(k2)),
+                  make_uint2(r3, __float_as_uint(k3)));
   ref = r0;
   return true;
 }
@@ -424,13 +466,27 @@ __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay
   const float sz = __uint_as_float(((n0.w >> 16) & 255u) << 23);
   const uint32_t n = n0.w >> 24;
   constexpr float kInf = __builtin_inff();
+  // Per axis, the words holding the planes the ray enters (near) and leaves
+  // (far) by: the min planes where the reciprocal direction is positive, the
+  // max planes where it is negative. The same planes the min / max of the
+  // two slab distances pick, chosen once for the four children.
+  const uint32_t mx = uint32_t(__float_as_int(r.inv.x) >> 31), my = uint32_t(__float_as_int(r.inv.y) >> 31),
+                 mz = uint32_t(__float_as_int(r.inv.z) >> 31);
+  const uint32_t nxw = (n1.w & mx) | (n1.x & ~mx), fxw = (n1.x & mx) | (n1.w & ~mx);
+  const uint32_t nyw = (n2.x & my) | (n1.y & ~my), fyw = (n1.y & my) | (n2.x & ~my);
+  const uint32_t nzw = (n2.y & mz) | (n1.z & ~mz), fzw = (n1.z & mz) | (n2.y & ~mz);
+  const float bm = below(best);
   float t[4];
   bool h[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    h[k] = box_hit(r, dequant(ox, n1.x, k, sx), dequant(ox, n1.w, k, sx), dequant(oy, n1.y, k, sy),
-                   dequant(oy, n2.x, k, sy), dequant(oz, n1.z, k, sz), dequant(oz, n2.y, k, sz), best, t[k]) &&
-           uint32_t(k) < n;
+    const float ax = (dequant(ox, nxw, k, sx) - r.o.x) * r.inv.x, bx = (dequant(ox, fxw, k, sx) - r.o.x) * r.inv.x;
+    const float ay = (dequant(oy, nyw, k, sy) - r.o.y) * r.inv.y, by = (dequant(oy, fyw, k, sy) - r.o.y) * r.inv.y;
+    const float az = (dequant(oz, nzw, k, sz) - r.o.z) * r.inv.z, bz = (dequant(oz, fzw, k, sz) - r.o.z) * r.inv.z;
+    const float l = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f)) * kWidenLo;
+    const float hh = fminf(fminf(bx, by), bz) * kWidenHi;
+    t[k] = l;
+    h[k] = l <= fminf(hh, bm) && uint32_t(k) < n;
   }
   use_now(rf);
   float k0 = h[0] ? t[0] : kInf, k1 = h[1] ? t[1] : kInf, k2 = h[2] ? t[2] : kInf, k3 = h[3] ? t[3] : kInf;
@@ -451,9 +507,8 @@ __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay
   cswap(k0, r0, k2, r2);
   cswap(k1, r1, k3, r3);
   cswap(k1, r1, k2, r2);
-  if (nhit > 3) stack.push(make_uint2(r3, __float_as_uint(k3)));
-  if (nhit > 2) stack.push(make_uint2(r2, __float_as_uint(k2)));
-  if (nhit > 1) stack.push(make_uint2(r1, __float_as_uint(k1)));
+  stack.push_last(nhit - 1, make_uint2(r1, __float_as_uint(k1)), make_uint2(r2, __float_as_uint(k2)),
+                  make_uint2(r3, __float_as_uint(k3)));
   ref = r0;
   return true;
 }
