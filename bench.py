@@ -11,7 +11,11 @@ Roofline (HBM-bound path, no dense contraction): algorithmic bytes per ray
   B_ray = 32 (ray in) + 16 (hit out) + 32 * N_node + 36 * N_tri
 where N_node / N_tri are the reference BVH's bounds / object tests per ray
 (muon/acceleration.cc:92,108), counted by the CPU restatement on a sample of
-the same rays; achieved = B_ray * rays / mean launch time, peak 8 TB/s.
+the same rays; achieved = B_ray * rays / mean launch time, peak 8 TB/s. The
+4-wide quantized tree touches far fewer bytes than the reference's binary
+tree, so this fraction can exceed 1; traffic_gbs / traffic_frac give the
+bytes the kernel actually moves past L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE
+per launch, read from the committed profile) over the same launch time.
 
 CPU baseline: the CPU restatement of the reference (same BVH, same
 per-primitive object-space tests) on a bounded sample of the same rays,
@@ -280,6 +284,9 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                # the bytes the kernel actually moves past L2 (profiled), per second of this run
+                "traffic_gbs": round(traffic / (mean_ms / 1e3) / 1e9, 1) if traffic else None,
+                "traffic_frac": round(traffic / (mean_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                 "bytes_per_ray": round(b_ray, 1), "ref_bounds_tests_per_ray": round(n_node, 2),
                 "ref_object_tests_per_ray": round(n_obj, 2),
                 "device_nodes_per_ray": round(dev_nodes / args.rays, 2),
