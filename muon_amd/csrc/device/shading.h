@@ -1,8 +1,13 @@
 // Device BRDFs, hemisphere sampling, lights and MIS helpers, restating
 // muon/materials.cc, muon/hemisphere_sampling.cc, muon/lighting.cc and the
-// sampling helpers of muon/integration.cc with the same formulas and
-// operation order. Transcendentals use the device math library, so these
-// stages match the reference statistically rather than bit for bit.
+// sampling helpers of muon/integration.cc with the same formulas. Where the
+// reference converts a cosine or tangent to an angle only to take its cosine,
+// sine or tangent again (the samplers' polar angle, GGX's D and G1) the
+// algebraic identity is used instead: cos(acos x) = x,
+// sin(acos x) = sqrt(1 - x^2), tan^2(acos x) = (1 - x^2) / x^2 and
+// (cos, sin)(atan(a / b)) = (b, a) / sqrt(a^2 + b^2). Those stages and the
+// device math library's transcendentals make these match the reference
+// statistically (same distributions and estimators), not bit for bit.
 #ifndef MUONX_DEVICE_SHADING_H_
 #define MUONX_DEVICE_SHADING_H_
 
@@ -20,25 +25,26 @@ __device__ __forceinline__ V3 rotate_to_frame(V3 s, V3 w) {
   return (s.x * u + s.y * v) + s.z * w;
 }
 
-__device__ __forceinline__ V3 spherical(float theta, float phi) {
-  float sp, cp, st, ct;
+// (cos phi sin theta, sin phi sin theta, cos theta) from the polar angle's
+// cosine and sine.
+__device__ __forceinline__ V3 spherical(float ct, float st, float phi) {
+  float sp, cp;
   sincosf(phi, &sp, &cp);
-  sincosf(theta, &st, &ct);
   return V3{cp * st, sp * st, ct};
 }
 
-// muon/hemisphere_sampling.cc:8-22
+// muon/hemisphere_sampling.cc:8-22 (theta = acos(r1))
 __device__ __forceinline__ V3 sample_hemisphere(V3 n, uint32_t key, uint32_t &ctr) {
   const float r1 = rng_uniform(key, ctr);
   const float r2 = rng_uniform(key, ctr);
-  return rotate_to_frame(spherical(acosf(r1), 2.0f * kPi * r2), n);
+  return rotate_to_frame(spherical(r1, sqrtf(gmax(1.0f - r1 * r1, 0.0f)), 2.0f * kPi * r2), n);
 }
 
-// muon/hemisphere_sampling.cc:24-38
+// muon/hemisphere_sampling.cc:24-38 (theta = acos(sqrt(r1)))
 __device__ __forceinline__ V3 sample_cosine(V3 n, uint32_t key, uint32_t &ctr) {
   const float r1 = rng_uniform(key, ctr);
   const float r2 = rng_uniform(key, ctr);
-  return rotate_to_frame(spherical(acosf(sqrtf(r1)), 2.0f * kPi * r2), n);
+  return rotate_to_frame(spherical(sqrtf(r1), sqrtf(1.0f - r1), 2.0f * kPi * r2), n);
 }
 
 struct Mat {
@@ -59,25 +65,24 @@ __device__ __forceinline__ Mat load_mat(const float *m) {
   return r;
 }
 
-// muon/materials.cc:131-141
+// muon/materials.cc:131-141: a^2 / (pi cos^4 (a^2 + tan^2)^2) with cos = h.n
+// clamped to [0, 1], written as a^2 / (pi (cos^2 (a^2 - 1) + 1)^2).
 __device__ __forceinline__ float ggx_d(V3 h, V3 n, float rough) {
-  const float ha = acosf(gclamp(dot(h, n), 0.0f, 1.0f));
+  const float c = gclamp(dot(h, n), 0.0f, 1.0f);
   const float a2 = rough * rough;
-  const float c = cosf(ha), t = tanf(ha);
-  const float c2 = c * c;
-  const float tt = a2 + t * t;
-  const float denom = kPi * (c2 * c2) * (tt * tt);
+  const float q = c * c * (a2 - 1.0f) + 1.0f;
+  const float denom = kPi * (q * q);
   return denom > 0.0f ? a2 / denom : 0.0f;
 }
 
-// muon/materials.cc:148-158
+// muon/materials.cc:148-158: 2 / (1 + sqrt(1 + a^2 tan^2)), tan^2 from v.n.
 __device__ __forceinline__ float smith_g1(V3 v, V3 n, float rough) {
   const float vn = dot(v, n);
   if (vn <= 0.0f) return 0.0f;
-  const float ang = acosf(gclamp(vn, 0.0f, 1.0f));
+  const float c = gmin(vn, 1.0f);
   const float a2 = rough * rough;
-  const float t = tanf(ang);
-  return 2.0f / (1.0f + sqrtf(1.0f + a2 * (t * t)));
+  const float t2 = (1.0f - c * c) / (c * c);
+  return 2.0f / (1.0f + sqrtf(1.0f + a2 * t2));
 }
 
 // BRDF::Sample: Lambertian muon/materials.cc:17-24, Phong :49-71, GGX :180-205.
@@ -88,11 +93,14 @@ __device__ __forceinline__ V3 brdf_sample(const Mat &m, V3 rd, V3 n, uint32_t ke
   const float r1 = rng_uniform(key, ctr);
   const float r2 = rng_uniform(key, ctr);
   if (m.brdf == MUONX_BRDF_PHONG) {
-    const float theta = acosf(powf(r1, 1.0f / (m.shininess + 1.0f)));
-    return rotate_to_frame(spherical(theta, 2.0f * kPi * r2), reflect(rd, n));
+    // theta = acos(r1^(1 / (s + 1)))
+    const float ct = powf(r1, 1.0f / (m.shininess + 1.0f));
+    return rotate_to_frame(spherical(ct, sqrtf(gmax(1.0f - ct * ct, 0.0f)), 2.0f * kPi * r2), reflect(rd, n));
   }
-  const float theta = atanf(m.roughness * sqrtf(r1) / sqrtf(1.0f - r1));
-  const V3 h = rotate_to_frame(spherical(theta, 2.0f * kPi * r2), n);
+  // theta = atan(a sqrt(r1) / sqrt(1 - r1)); 1 - r1 > 0 (r1 < 1)
+  const float b = sqrtf(1.0f - r1), a = m.roughness * sqrtf(r1);
+  const float inv = 1.0f / sqrtf(b * b + a * a);
+  const V3 h = rotate_to_frame(spherical(b * inv, a * inv, 2.0f * kPi * r2), n);
   return reflect(rd, h);
 }
 
