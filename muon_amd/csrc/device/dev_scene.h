@@ -18,9 +18,10 @@
 //   w0 = child min.x[0..3]  w1 = child max.x[0..3]
 //   w2 = child min.y[0..3]  w3 = child max.y[0..3]
 //   w4 = child min.z[0..3]  w5 = child max.z[0..3]
-//   w6 = child refs[0..3] (node index, or leaf bit | first primitive)
+//   w6 = child refs[0..3] (node index, or a leaf reference, below)
 //   w7 = (child count, 0, 0, 0)
-// The same leaves (primitive ranges) as the binary tree.
+// Leaves are primitive ranges of the binary tree: its leaves, or (when leaf
+// references carry counts) whole binary subtrees of a few primitives.
 //
 // Quantized wide BVH: the same nodes in 64 B = 4 x uint4, child planes as
 // 8-bit offsets from a per-node origin in steps of a per-axis power of two:
@@ -46,6 +47,14 @@ namespace muonx {
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kCountMask = 0x1fffffffu;
+// Leaf references of the wide trees when the scene's leaves carry their
+// primitive count (DevSceneView::leaf_counts): leaf bit | (count - 1) << 24 |
+// first primitive, for scenes of fewer than 2^24 primitives and leaves of at
+// most 128. Otherwise leaf bit | first primitive, and a leaf ends at the
+// primitive flagged last (as in the binary tree).
+constexpr uint32_t kLeafCountShift = 24;
+constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
+constexpr uint32_t kLeafMaxCount = 128;
 constexpr uint32_t kSphereBit = 0x80000000u;
 
 // Material record, 16 floats: ambient3 diffuse3 specular3 emission3
@@ -74,6 +83,7 @@ struct DevSceneView {
   uint32_t bvh_depth;
   uint32_t num_wnodes;
   uint32_t wstack;          // bound on stack entries of a wide traversal
+  uint32_t leaf_counts;     // 1: wide leaf references carry their primitive count
 };
 
 }  // namespace muonx

@@ -328,10 +328,16 @@ __device__ __forceinline__ bool pop_next(Stack &stack, float best, uint32_t &ref
 }
 
 // Tests the primitives of the leaf at `ref`; returns true if ANY found a hit.
-template <bool ANY, bool COUNT>
+// A wide-tree leaf (WIDE) whose reference carries a count tests that many
+// primitives; otherwise the leaf ends at the primitive flagged last.
+template <bool ANY, bool COUNT, bool WIDE = false>
 __device__ __forceinline__ bool leaf_step(const DevSceneView &s, const TraceRay &r, float &best, uint32_t ref,
                                           ObjCache &oc, int32_t &hit_prim, TraceCounters &cnt) {
-  uint32_t i = ref & ~kLeafBit;
+  uint32_t i = ref & ~kLeafBit, left = 0;
+  if (WIDE && s.leaf_counts) {
+    i = ref & kLeafFirstMask;
+    left = ((ref >> kLeafCountShift) & (kLeafMaxCount - 1u)) + 1u;
+  }
   while (true) {
     float dist;
     uint32_t flags;
@@ -341,7 +347,7 @@ __device__ __forceinline__ bool leaf_step(const DevSceneView &s, const TraceRay 
       hit_prim = int32_t(i);
       if (ANY) return true;
     }
-    if (flags & 1u) return false;
+    if (left ? --left == 0u : (flags & 1u) != 0u) return false;
     ++i;
   }
 }
@@ -541,7 +547,7 @@ __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const Tra
         }
       }
       if (!alive) break;
-      if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
+      if (leaf_step<ANY, COUNT, true>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
       if (!pop_next(stack, best, ref)) break;
     }
   } else {
@@ -549,7 +555,7 @@ __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const Tra
       if (!(ref & kLeafBit)) {
         if (fast_step<NODES, COUNT>(s, r, best, stack, ref, cnt)) continue;
       } else {
-        if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
+        if (leaf_step<ANY, COUNT, true>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
       }
       if (!pop_next(stack, best, ref)) break;
     }
@@ -590,7 +596,7 @@ struct LaneTrace {
     while (!(ref & kLeafBit)) {
       if (!fast_step<WIDTH, COUNT>(s, r, best, st, ref, cnt) && !pop_next(st, best, ref)) return true;
     }
-    if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit, cnt)) return true;
+    if (leaf_step<ANY, COUNT, WIDTH != kNodesBinary>(s, r, best, ref, oc, hit, cnt)) return true;
     return !pop_next(st, best, ref);
   }
 };

@@ -163,3 +163,53 @@ def test_icosphere_full_microbench_invariants(icosphere, width):
     idx = rng.choice(o.shape[0], 4096, replace=False)
     pr, tr, _, _, _ = orc.intersect(o[idx], d[idx])
     assert not classify_mismatches(orc, o[idx], d[idx], prim[idx], t[idx], pr, tr, rel=1e-5)
+
+
+def _stacked_leaf_text(n_stack, n_rand=200, seed=3):
+    """n_stack copies of one triangle (identical bounds: the SAH build keeps
+    them in one leaf) beside n_rand small random triangles."""
+    rng = np.random.default_rng(seed)
+    verts, tris = [], []
+    for _ in range(n_stack):
+        b = len(verts)
+        verts += [(0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)]
+        tris.append((b, b + 1, b + 2))
+    for _ in range(n_rand):
+        c = rng.uniform(-3, 3, 3)
+        b = len(verts)
+        verts += [tuple(float(x) for x in c + rng.uniform(-0.3, 0.3, 3)) for _ in range(3)]
+        tris.append((b, b + 1, b + 2))
+    lines = ["camera 0 0 5 0 0 0 0 1 0 45", f"maxverts {len(verts)}"]
+    lines += ["vertex %r %r %r" % v for v in verts]
+    lines += ["tri %d %d %d" % t for t in tris]
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("n_stack", [100, 150], ids=["leaf100-counted", "leaf150-flagged"])
+def test_large_leaf(ctx, width, oracle, n_stack):
+    """A leaf of 100 primitives (the wide tree's leaf references carry counts
+    up to 128) and one of 150 (leaf references fall back to the end-of-leaf
+    flag): fast-mode ids equal the oracle's outside ties, any-hit flags equal
+    but for edge grazes (fraction below 1e-4, as for the reference scenes)."""
+    text = _stacked_leaf_text(n_stack)
+    hs = HostScene.from_text(text)
+    assert int(hs.arrays()["node_count"].max()) == n_stack
+    ds = DeviceScene(ctx, hs)
+    orc = oracle.load_text(text)
+    rng = np.random.default_rng(17)
+    lo, hi = scene_bounds(hs)
+    o, d = random_rays(rng, 1 << 15, lo, hi)
+    # half of the rays aimed at points of the stacked triangle
+    k = o.shape[0] // 2
+    uv = rng.random((k, 2), dtype=np.float32) * np.float32(0.5)
+    tgt = np.stack([uv[:, 0], uv[:, 1], np.zeros(k, np.float32)], axis=1)
+    dd = tgt - o[:k]
+    d[:k] = (dd / np.linalg.norm(dd, axis=1, keepdims=True)).astype(np.float32)
+    tmax = np.full(o.shape[0], np.inf, np.float32)
+    prim, t, occ = _query(ds, o, d, tmax)
+    _, hits = _check(orc, o, d, prim, t)
+    assert hits > 1000
+    stack_hits = int(((prim >= 0) & (prim < n_stack)).sum())
+    assert stack_hits > 1000
+    oref, _ = orc.occluded(o, d, tmax)
+    assert (occ != oref).mean() < 1e-4
