@@ -4,16 +4,16 @@ checkout is absent.
 
 These are data files of the reference's test suite (test/*.muon scenes and
 test/testdata/*.png goldens): inputs and expected outputs, not source code.
-The 3.7 MB dragon scene is left out; the dragon test is statistical and
-covered by the other MAE-mode scenes.
+The 3.7 MB dragon scene is stored gzip-compressed (dragon.muon.gz, 1.3 MB).
 
 Usage: python tests/golden/import_reference_fixtures.py [/path/to/reference]
 """
+import gzip
 import os
 import shutil
 import sys
 
-SKIP = {"dragon.muon", "dragon_golden.png", "dragon_truth.png"}
+COMPRESS = {"dragon.muon"}
 
 
 def main() -> None:
@@ -25,9 +25,14 @@ def main() -> None:
     for sub in ("", "testdata"):
         d = os.path.join(src, sub)
         for name in sorted(os.listdir(d)):
-            if name in SKIP or not (name.endswith(".muon") or name.endswith(".png")):
+            if not (name.endswith(".muon") or name.endswith(".png")):
                 continue
-            shutil.copyfile(os.path.join(d, name), os.path.join(dst, name))
+            if name in COMPRESS:
+                with open(os.path.join(d, name), "rb") as f, \
+                        gzip.GzipFile(os.path.join(dst, name + ".gz"), "wb", compresslevel=9, mtime=0) as g:
+                    g.write(f.read())
+            else:
+                shutil.copyfile(os.path.join(d, name), os.path.join(dst, name))
             n += 1
     print(f"copied {n} fixtures into {dst}")
 

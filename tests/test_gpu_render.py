@@ -7,6 +7,7 @@
   pairs (|MAE(out,truth) - MAE(golden,truth)| <= tol, test/diff_test.sh),
   and relMSE against a converged GPU image within 2% of the oracle's
   relMSE at equal spp (criterion (c))."""
+import gzip
 import os
 
 import numpy as np
@@ -62,6 +63,21 @@ def test_deterministic_film_matches_oracle_and_golden(ctx, oracle, ref_dir, name
         assert stats.secondary_rays == ref_st[1] and stats.shadow_rays == ref_st[6]
 
 
+def test_dragon_vertex_normals_match_oracle(ctx, oracle, ref_dir):
+    """The 100k-triangle dragon (compute_vertex_normals on) through the
+    normals integrator at one sample per pixel: every pixel's interpolated
+    world normal within 1e-5 of the oracle's, film bits equal."""
+    text = _scene_text(ref_dir, "dragon").replace("integrator pathtracer", "integrator normals")
+    text = text.replace("pixel_samples 64", "pixel_samples 1")
+    hs = HostScene.from_text(text, default_seed=3)
+    acc, stats = gpu_render(ctx, hs)
+    ref_acc, _, ref_st, _ = oracle.load_text(text, seed=3).render(1)
+    assert stats.primary_rays == ref_st[0]
+    assert (np.abs(acc - ref_acc) <= 1e-5 * np.maximum(np.abs(ref_acc), 1.0)).all()
+    assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32))
+    assert (acc.sum(axis=2) > 0).mean() > 0.3
+
+
 def test_albedo_integrator(ctx, oracle, ref_dir):
     text = open(os.path.join(ref_dir, "cornell_normals.muon")).read().replace("integrator normals", "integrator albedo")
     acc, _ = gpu_render(ctx, HostScene.from_text(text))
@@ -70,12 +86,20 @@ def test_albedo_integrator(ctx, oracle, ref_dir):
 
 
 MAE_TOL = {"sphere": 5e-4, "cornell_phong": 2e-3, "cornell_brdf": 2e-3, "ggx": 2e-3, "mis": 2e-3,
-           "partial_direct": 5e-4, "partial_indirect": 1e-3}
+           "partial_direct": 5e-4, "partial_indirect": 1e-3, "dragon": 2e-3}
+
+
+def _scene_text(ref_dir, name):
+    path = os.path.join(ref_dir, name + ".muon")
+    if os.path.exists(path):
+        return open(path).read()
+    with gzip.open(path + ".gz", "rt") as f:  # the 100k-triangle dragon is stored compressed
+        return f.read()
 
 
 @pytest.mark.parametrize("name", sorted(MAE_TOL))
 def test_path_tracer_statistical_golden(ctx, ref_dir, name):
-    hs = HostScene.from_file(os.path.join(ref_dir, name + ".muon"), default_seed=5)
+    hs = HostScene.from_text(_scene_text(ref_dir, name), default_seed=5)
     acc, _ = gpu_render(ctx, hs)
     rgb = Film(acc, hs.params.spp, hs.params.gamma).rgb8()
     truth = _png(ref_dir, name + "_truth.png")

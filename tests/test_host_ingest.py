@@ -67,6 +67,17 @@ def test_reference_scene_ingest_matches_oracle(oracle, ref_dir, name):
         assert host.params.seed == orc.pathtracer_seed()
 
 
+def test_dragon_ingest_matches_oracle(oracle, ref_dir):
+    """The reference's 100k-triangle dragon scene (stored compressed)."""
+    import gzip
+    with gzip.open(os.path.join(ref_dir, "dragon.muon.gz"), "rt") as f:
+        text = f.read()
+    host = HostScene.from_text(text, default_seed=7)
+    assert host.desc.num_prims == 100008
+    _compare(host, oracle.load_text(text, seed=7))
+    assert host.params.seed == oracle.load_text(text, seed=7).pathtracer_seed()
+
+
 @pytest.mark.parametrize("part", [0, 1, 2])
 def test_partition_strategies_match_oracle(oracle, ref_dir, part):
     path = os.path.join(ref_dir, "spheres_raytrace.muon")
