@@ -25,6 +25,7 @@ extern "C" {
 #define MUONX_ERR_IO (-4)        /* file could not be read */
 #define MUONX_ERR_STATE (-5)     /* call not valid in the object's state */
 #define MUONX_ERR_CAPACITY (-6)  /* request exceeds an admission limit */
+#define MUONX_ERR_CANCELLED (-7) /* the job was cancelled before it started rendering */
 
 /* Integrator selection (muon/parser.cc:330-355). */
 #define MUONX_INTEGRATOR_NORMALS 0
@@ -313,9 +314,16 @@ int muonx_server_create(muonx_ctx *ctx, const muonx_engine_config *cfg, muonx_se
  * (film above the budget, too many shadow rays per vertex) are rejected
  * with MUONX_ERR_CAPACITY. */
 int muonx_server_submit(muonx_server *srv, const muonx_job *job, uint64_t *ticket);
-/* Waits up to timeout_ms (< 0: forever) for a ticket. Returns MUONX_OK and
- * fills `result` (the ticket is then consumed), MUONX_PENDING on timeout. */
+/* Waits up to timeout_ms (< 0: forever, 0: poll) for a ticket. Returns
+ * MUONX_OK and fills `result` (the ticket is then consumed), MUONX_PENDING
+ * on timeout, MUONX_ERR_CANCELLED (result filled, ticket consumed) for a job
+ * that was cancelled before it started rendering, or the job's error. */
 int muonx_server_wait(muonx_server *srv, uint64_t ticket, int64_t timeout_ms, muonx_job_result *result);
+/* Cancels a submitted job that has not started rendering: it is dropped
+ * (its film is not written) and its wait returns MUONX_ERR_CANCELLED. A job
+ * already admitted to the GPU finishes normally. Returns MUONX_OK when the
+ * request is recorded, MUONX_ERR_INVALID for an unknown or collected ticket. */
+int muonx_server_cancel(muonx_server *srv, uint64_t ticket);
 
 typedef struct muonx_server_stats {
   uint64_t submitted, completed, pending, in_flight;
@@ -324,6 +332,7 @@ typedef struct muonx_server_stats {
   uint64_t film_bytes_in_use, film_budget_bytes;
   uint32_t path_slots;
   int32_t shadow_per_vertex;
+  uint64_t cancelled;           /* jobs dropped by muonx_server_cancel */
 } muonx_server_stats;
 int muonx_server_get_stats(muonx_server *srv, muonx_server_stats *out);
 /* Finishes every submitted job, then stops the thread and frees the engine. */

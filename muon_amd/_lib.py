@@ -89,10 +89,12 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("pending", C.c_uint64),
                 ("in_flight", C.c_uint64), ("iterations", C.c_uint64), ("primary_rays", C.c_uint64),
                 ("secondary_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("film_bytes_in_use", C.c_uint64),
-                ("film_budget_bytes", C.c_uint64), ("path_slots", C.c_uint32), ("shadow_per_vertex", C.c_int32)]
+                ("film_budget_bytes", C.c_uint64), ("path_slots", C.c_uint32), ("shadow_per_vertex", C.c_int32),
+                ("cancelled", C.c_uint64)]
 
 
 MUONX_PENDING = 1
+MUONX_ERR_CANCELLED = -7
 
 P = C.c_void_p
 # (name, restype, argtypes): every entry point declared in include/muonx.h
@@ -123,6 +125,7 @@ SIGNATURES = [
     ("muonx_server_create", C.c_int, [P, C.POINTER(EngineConfig), C.POINTER(P)]),
     ("muonx_server_submit", C.c_int, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     ("muonx_server_wait", C.c_int, [P, C.c_uint64, C.c_int64, C.POINTER(JobResult)]),
+    ("muonx_server_cancel", C.c_int, [P, C.c_uint64]),
     ("muonx_server_get_stats", C.c_int, [P, C.POINTER(ServerStats)]),
     ("muonx_server_destroy", None, [P]),
     ("muonx_film_to_rgb8", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),

@@ -165,6 +165,7 @@ struct muonx_server {
   std::deque<Incoming> incoming;
   std::unordered_map<uint64_t, muonx_job_result> results;  // finished, not yet collected
   std::unordered_map<uint64_t, muonx_scene *> outstanding;  // submitted, not finished
+  std::vector<uint64_t> cancels;  // cancel requests for jobs handed to the scheduler
   uint64_t next_ticket = 1;
   bool stop = false;
   int fatal = MUONX_OK;
@@ -173,6 +174,21 @@ struct muonx_server {
 };
 
 namespace {
+
+// Records a job dropped before admission (caller holds srv->mu).
+void PostCancelled(muonx_server *srv, uint64_t ticket, uint64_t submit_ns) {
+  muonx_job_result r{};
+  r.status = MUONX_ERR_CANCELLED;
+  r.submit_ns = submit_ns;
+  r.done_ns = NowNs();
+  srv->results[ticket] = r;
+  auto it = srv->outstanding.find(ticket);
+  if (it != srv->outstanding.end()) {
+    it->second->inflight.fetch_sub(1);
+    srv->outstanding.erase(it);
+  }
+  ++srv->stats.cancelled;
+}
 
 void ServerLoop(muonx_server *srv) {
   if (hipSetDevice(srv->ctx->device) != hipSuccess) {
@@ -185,6 +201,8 @@ void ServerLoop(muonx_server *srv) {
   Scheduler *s = srv->sched;
   std::vector<SchedJob> finished;
   std::vector<muonx_server::Incoming> take;
+  std::vector<uint64_t> cancels;
+  std::vector<std::pair<uint64_t, uint64_t>> dropped;  // (ticket, submit_ns)
   while (true) {
     {
       std::unique_lock<std::mutex> lock(srv->mu);
@@ -192,13 +210,24 @@ void ServerLoop(muonx_server *srv) {
       if (srv->stop && srv->incoming.empty() && s->Idle()) break;
       take.assign(srv->incoming.begin(), srv->incoming.end());
       srv->incoming.clear();
+      cancels.swap(srv->cancels);
+      srv->cancels.clear();
     }
     int rc = MUONX_OK;
     for (const auto &in : take)
       if (rc == MUONX_OK) rc = s->Submit(in.job, in.ticket, in.submit_ns);
+    // cancel requests: jobs still waiting for admission are dropped; the
+    // others (admitted or finished) are left to complete
+    dropped.clear();
+    for (const uint64_t t : cancels) {
+      uint64_t submit_ns = 0;
+      if (s->Cancel(t, &submit_ns)) dropped.emplace_back(t, submit_ns);
+    }
+    cancels.clear();
     finished.clear();
     if (rc == MUONX_OK) rc = s->Step(&finished);
     std::lock_guard<std::mutex> lock(srv->mu);
+    for (const auto &d : dropped) PostCancelled(srv, d.first, d.second);
     for (const SchedJob &j : finished) {
       srv->results[j.ticket] = j.result;
       srv->outstanding.erase(j.ticket);
@@ -227,7 +256,7 @@ void ServerLoop(muonx_server *srv) {
       srv->cv_done.notify_all();
       return;
     }
-    if (!finished.empty()) srv->cv_done.notify_all();
+    if (!finished.empty() || !dropped.empty()) srv->cv_done.notify_all();
   }
 }
 
@@ -308,10 +337,39 @@ extern "C" int muonx_server_wait(muonx_server *srv, uint64_t ticket, int64_t tim
   const muonx_job_result r = srv->results[ticket];
   srv->results.erase(ticket);
   if (result) *result = r;
+  if (r.status == MUONX_ERR_CANCELLED) {
+    SetError("muonx_server_wait: ticket " + std::to_string(ticket) + " was cancelled");
+    return r.status;
+  }
   if (r.status != MUONX_OK) {
     SetError("job failed: " + srv->fatal_msg);
     return r.status;
   }
+  return MUONX_OK;
+}
+
+extern "C" int muonx_server_cancel(muonx_server *srv, uint64_t ticket) {
+  if (!srv) {
+    SetError("muonx_server_cancel: null server");
+    return MUONX_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(srv->mu);
+  if (srv->results.count(ticket)) return MUONX_OK;  // finished: nothing to cancel
+  if (!srv->outstanding.count(ticket)) {
+    SetError("muonx_server_cancel: unknown or already collected ticket " + std::to_string(ticket));
+    return MUONX_ERR_INVALID;
+  }
+  for (auto it = srv->incoming.begin(); it != srv->incoming.end(); ++it) {
+    if (it->ticket == ticket) {  // not yet handed to the scheduler
+      const uint64_t submit_ns = it->submit_ns;
+      srv->incoming.erase(it);
+      PostCancelled(srv, ticket, submit_ns);
+      srv->cv_done.notify_all();
+      return MUONX_OK;
+    }
+  }
+  srv->cancels.push_back(ticket);  // the server thread drops it if it is still pending
+  srv->cv_work.notify_one();
   return MUONX_OK;
 }
 

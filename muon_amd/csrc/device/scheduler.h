@@ -63,6 +63,9 @@ class Scheduler {
   int Validate(const muonx_job &job) const;
   // Queues a validated job; `submit_ns` is its arrival time.
   int Submit(const muonx_job &job, uint64_t ticket, uint64_t submit_ns);
+  // Removes a job that is still waiting for admission; false if it is not
+  // pending (admitted, finished or unknown). `submit_ns` gets its arrival.
+  bool Cancel(uint64_t ticket, uint64_t *submit_ns);
   // One round: admits what fits, launches `iterations_per_poll` engine
   // iterations, harvests the previous round's done counters and finished
   // resolves. Jobs that completed are appended to `finished`.

@@ -143,6 +143,17 @@ int Scheduler::Validate(const muonx_job &job) const {
   return MUONX_OK;
 }
 
+bool Scheduler::Cancel(uint64_t ticket, uint64_t *submit_ns) {
+  for (auto it = pending_.begin(); it != pending_.end(); ++it) {
+    if (it->ticket == ticket) {
+      if (submit_ns) *submit_ns = it->result.submit_ns;
+      pending_.erase(it);
+      return true;
+    }
+  }
+  return false;
+}
+
 int Scheduler::Submit(const muonx_job &job, uint64_t ticket, uint64_t submit_ns) {
   SchedJob j;
   j.ticket = ticket;

@@ -91,14 +91,23 @@ class Server:
         return t.value
 
     def wait(self, ticket, timeout_ms=-1):
-        """Returns (accum, JobResult), or None on timeout."""
+        """Returns (accum, JobResult), or None on timeout; (None, JobResult)
+        with status MUONX_ERR_CANCELLED for a job cancelled before it
+        started rendering."""
         r = _lib.JobResult()
         rc = lib.muonx_server_wait(self.h, ticket, timeout_ms, C.byref(r))
         if rc == _lib.MUONX_PENDING:
             return None
         accum = self._outputs.pop(ticket, None)
+        if rc == _lib.MUONX_ERR_CANCELLED:
+            return None, _result(r)
         check(rc)
         return accum, _result(r)
+
+    def cancel(self, ticket):
+        """Drops a job that has not started rendering (muonx_server_cancel);
+        a job already rendering finishes normally."""
+        check(lib.muonx_server_cancel(self.h, ticket))
 
     def stats(self):
         s = _lib.ServerStats()

@@ -135,6 +135,42 @@ def test_server_results_equal_alone_under_load(ctx, resident):
         assert np.array_equal(acc.cpu().numpy().view(np.uint32), r.view(np.uint32)), f"request {i}"
 
 
+def test_server_cancel(ctx, resident):
+    """The film budget holds one 512x512 film: B and C wait while A renders.
+    Cancelling B drops it (its film is never written, wait reports
+    MUONX_ERR_CANCELLED); cancelling A, already rendering, changes nothing;
+    C runs after A. A and C equal their renders alone."""
+    hs, ds = resident["ico"]
+    p = params_of(hs, width=512, height=512, spp=64, max_depth=8)
+    q = params_of(hs, width=512, height=512, spp=4, max_depth=8, seed=3)
+    ref_a, ref_c = alone(ds, p), alone(ds, q)
+    cfg = engine_config(max_jobs=4, film_budget_bytes=512 * 512 * 24 + 512 * 512 * 12)
+    with Server(ctx, cfg) as srv:
+        ta = srv.submit(ds, p)
+        sentinel = torch.full((512, 512, 3), 7.0, dtype=torch.float32, device="cuda:0")
+        tb = srv.submit(ds, q, accum=sentinel)
+        tc = srv.submit(ds, q)
+        srv.cancel(tb)
+        srv.cancel(ta)  # rendering (or about to): completes normally
+        acc_b, res_b = srv.wait(tb, timeout_ms=60000)
+        assert acc_b is None and res_b.status == _lib.MUONX_ERR_CANCELLED
+        acc_a, res_a = srv.wait(ta, timeout_ms=60000)
+        acc_c, res_c = srv.wait(tc, timeout_ms=60000)
+        st = srv.stats()
+        with pytest.raises(MuonxError) as e:
+            srv.cancel(tb)  # collected
+        assert e.value.code == -1
+        with pytest.raises(MuonxError) as e:
+            srv.cancel(10 ** 9)  # never issued
+        assert e.value.code == -1
+    torch.cuda.synchronize()
+    assert res_a.status == 0 and res_c.status == 0
+    assert np.array_equal(acc_a.cpu().numpy().view(np.uint32), ref_a.view(np.uint32))
+    assert np.array_equal(acc_c.cpu().numpy().view(np.uint32), ref_c.view(np.uint32))
+    assert bool((sentinel == 7.0).all())
+    assert st["cancelled"] == 1 and st["completed"] == 2 and st["film_bytes_in_use"] == 0
+
+
 def test_server_rejects_what_can_never_fit(ctx, resident):
     hs, ds = resident["cornell"]
     cfg = engine_config(path_slots=4096, max_jobs=4, shadow_per_vertex=1, film_budget_bytes=64 * 64 * 24)
