@@ -48,7 +48,8 @@ struct JobDev {
   uint32_t seed, key;
   int min_depth, cutoff, nee, is_mode, rr, light_samples, strata, per_section, K;
   unsigned long long first_item;  // global index of the job's first sample
-  unsigned long long nitems;      // W * H * spp
+  unsigned long long nitems;      // W * H * (samples of this slice)
+  uint32_t sample_base;           // first sample index of the slice
   unsigned long long *film;       // W * H * 3 fixed-point sums
   unsigned long long *done;       // finished samples
 };
@@ -107,11 +108,15 @@ struct EngArgs {
   uint32_t *s_queue;
 };
 
+// One engine job: samples [sample_begin, sample_begin + sample_count) of
+// every pixel of a render (a slice; a render may be admitted as several).
 struct JobSpec {
   uint32_t scene_slot;
   muonx_render_params params;
   uint32_t key;
-  unsigned long long *film;  // npix * 3 fixed-point sums, zeroed by Admit
+  unsigned long long *film;  // npix * 3 fixed-point sums, zeroed by Admit when clear_film
+  uint32_t sample_begin = 0, sample_count = 0;
+  bool clear_film = true;
 };
 
 class Engine {
@@ -127,7 +132,8 @@ class Engine {
   int AddScene(muonx_scene *scene, uint32_t *slot);
   // Shadow rays one vertex of a job generates; admission requires <= kmax.
   static int ShadowRaysPerVertex(const muonx_scene *scene, const muonx_render_params &p);
-  // Admits a job (film and done counter are zeroed on the engine stream).
+  // Admits a job (its done counter, and with clear_film its film, are
+  // zeroed on the engine stream).
   int Admit(const JobSpec &job, uint64_t *seq);
   // Advances the admission window past finished jobs (all of their samples
   // generated and retired).

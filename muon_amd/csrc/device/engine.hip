@@ -463,6 +463,8 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
 }
 
 // ---------------------------------------------------------------- retire
+constexpr uint32_t kDoneTable = 64;  // per-block done-count entries (a power of two)
+
 // Adds each shaded path's NEE and MIS terms in a fixed order. A continuing
 // path's radiance moves to its place in the next queue; a finished path adds
 // it to the job's film and counts towards the job's done counter.
@@ -471,8 +473,13 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
   const PathState &ps = a.st[cur];
   const PathState &pn = a.st[cur ^ 1];
   const size_t P = a.capacity;
-  uint32_t carry_job = 0xffffffffu;  // same in every lane
-  uint64_t carry_n = 0;
+  __shared__ uint32_t tjob[kDoneTable];
+  __shared__ unsigned long long tcnt[kDoneTable];
+  for (uint32_t k = threadIdx.x; k < kDoneTable; k += blockDim.x) {
+    tjob[k] = 0xffffffffu;
+    tcnt[k] = 0;
+  }
+  __syncthreads();
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t i = base + threadIdx.x;
     bool fin = false;
@@ -516,26 +523,28 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
         fin = true;
       }
     }
-    // done counters: the wavefront carries (job, count) across iterations
-    // and adds it when the job changes, so one job costs about one atomic per
-    // wavefront instead of one per iteration (the host reads them after the
-    // kernel)
+    // done counters: per block, counts gather in a small LDS table keyed by
+    // job slot and reach the job's counter with one atomic per (block, job)
+    // at the end (the queue holds the paths of several jobs in block-sized
+    // runs; one atomic per wavefront iteration on a handful of addresses
+    // would serialise, and the host reads the counters after the kernel)
     uint64_t pending = __ballot(fin);
     while (pending) {
       const int leader = __ffsll((unsigned long long)pending) - 1;
       const uint32_t jl = uint32_t(__shfl(int(js), leader));
       const uint64_t same = __ballot(fin && js == jl);
-      if (jl == carry_job) {
-        carry_n += uint64_t(__popcll(same));
-      } else {
-        if (carry_n && __lane_id() == 0) atomicAdd(a.jobs[carry_job].done, (unsigned long long)carry_n);
-        carry_job = jl;
-        carry_n = uint64_t(__popcll(same));
+      if (__lane_id() == 0) {
+        const uint32_t h = jl & (kDoneTable - 1);
+        const uint32_t prev = atomicCAS(&tjob[h], 0xffffffffu, jl);
+        if (prev == 0xffffffffu || prev == jl) atomicAdd(&tcnt[h], (unsigned long long)__popcll(same));
+        else atomicAdd(a.jobs[jl].done, (unsigned long long)__popcll(same));  // another job holds the entry
       }
       pending &= ~same;
     }
   }
-  if (carry_n && __lane_id() == 0) atomicAdd(a.jobs[carry_job].done, (unsigned long long)carry_n);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < kDoneTable; k += blockDim.x)
+    if (tcnt[k]) atomicAdd(a.jobs[tjob[k]].done, tcnt[k]);
 }
 
 // ---------------------------------------------------------------- plan / fill
@@ -579,7 +588,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
     const uint32_t js = uint32_t(l % a.max_jobs);
     const JobDev &job = a.jobs[js];
     const unsigned long long local = item - job.first_item;
-    const uint32_t sample = uint32_t(local / job.npix);
+    const uint32_t sample = job.sample_base + uint32_t(local / job.npix);
     const uint32_t pixel = uint32_t(local % job.npix);
     const int px = int(pixel % uint32_t(job.cam.width)), py = int(pixel / uint32_t(job.cam.width));
     float x, y;
@@ -884,12 +893,17 @@ int Engine::Admit(const JobSpec &js, uint64_t *seq) {
   j.light_samples = std::max(1, p.light_samples);
   LightGrid(p, &j.strata, &j.per_section);
   j.K = (p.nee == MUONX_NEE_OFF || scene->view.num_lights == 0) ? 0 : K;
+  if (js.sample_count == 0 || uint64_t(js.sample_begin) + js.sample_count > uint64_t(p.spp)) {
+    SetError("engine: sample slice outside [0, spp)");
+    return MUONX_ERR_INVALID;
+  }
   j.first_item = total_items_;
-  j.nitems = uint64_t(j.npix) * uint64_t(p.spp);
+  j.nitems = uint64_t(j.npix) * uint64_t(js.sample_count);
+  j.sample_base = js.sample_begin;
   j.film = js.film;
   j.done = done_.as<unsigned long long>() + slot;
   h_first_[slot] = j.first_item;
-  MX_HIP(hipMemsetAsync(js.film, 0, size_t(j.npix) * 3 * sizeof(unsigned long long), stream_));
+  if (js.clear_film) MX_HIP(hipMemsetAsync(js.film, 0, size_t(j.npix) * 3 * sizeof(unsigned long long), stream_));
   MX_HIP(hipMemsetAsync(j.done, 0, sizeof(unsigned long long), stream_));
   MX_HIP(hipMemcpyAsync(jobs_.as<JobDev>() + slot, &j, sizeof(JobDev), hipMemcpyHostToDevice, stream_));
   MX_HIP(hipMemcpyAsync(job_first_.as<unsigned long long>() + slot, &h_first_[slot], sizeof(unsigned long long), hipMemcpyHostToDevice,

@@ -41,14 +41,21 @@ struct SchedJob {
   uint64_t ticket = 0;
   muonx_job spec{};
   uint32_t scene_slot = 0;
-  uint64_t seq = 0;
+  uint32_t npix = 0;
   uint64_t film_off = 0, film_bytes = 0;
-  uint64_t nitems = 0;
-  uint64_t admit_snap = 0;  // first done-snapshot taken after admission
-  uint64_t seen_done = 0;   // finished samples at the last snapshot
+  uint32_t next_sample = 0;  // samples handed to the engine so far
+  uint32_t open_slices = 0;  // of those, slices not finished yet
   int state = 0;  // kPending, kActive, kResolving, kDone
   muonx_job_result result{};
   hipEvent_t ev = nullptr;  // resolve finished
+};
+
+// Consecutive samples of one job admitted as one engine job.
+struct Slice {
+  uint64_t ticket = 0;      // its job
+  uint64_t nitems = 0;      // pixels x samples
+  uint64_t admit_snap = 0;  // first done-snapshot taken after admission
+  uint64_t seen_done = 0;   // finished samples at the last snapshot
 };
 
 class Scheduler {
@@ -66,13 +73,22 @@ class Scheduler {
   // Removes a job that is still waiting for admission; false if it is not
   // pending (admitted, finished or unknown). `submit_ns` gets its arrival.
   bool Cancel(uint64_t ticket, uint64_t *submit_ns);
-  // One round: admits what fits, launches `iterations_per_poll` engine
+  // One round: admits work, launches `iterations_per_poll` engine
   // iterations, harvests the previous round's done counters and finished
   // resolves. Jobs that completed are appended to `finished`.
+  //
+  // Admission: a job enters in arrival order once its film fits the film
+  // budget; its samples go to the engine in slices of about a quarter of the
+  // path capacity, and the engine is given new slices only while fewer than
+  // 1.5 capacities of samples are unfinished. A newly admitted job's first
+  // slice comes before further slices of running jobs (which continue
+  // oldest first), so a small request waits behind about one queue fill,
+  // not behind the whole of a large one, while large requests still finish
+  // in arrival order. Images do not depend on the slicing.
   int Step(std::vector<SchedJob> *finished);
-  bool Idle() const { return pending_.empty() && active_.empty(); }
+  bool Idle() const { return pending_.empty() && jobs_.empty(); }
   size_t pending() const { return pending_.size(); }
-  size_t in_flight() const { return active_.size(); }
+  size_t in_flight() const { return jobs_.size(); }
   Engine *engine() const { return eng_; }
   uint64_t iterations() const { return iterations_; }
   const unsigned long long *rays() const { return rays_; }
@@ -82,7 +98,8 @@ class Scheduler {
 
  private:
   Scheduler() = default;
-  int AdmitPending();
+  int AdmitWork();
+  int AdmitSlice(SchedJob &j);
   int Harvest(int b, std::vector<SchedJob> *finished);
   int FinishResolves(std::vector<SchedJob> *finished, bool wait_all);
   void AdvanceWindow();
@@ -92,7 +109,9 @@ class Scheduler {
   Engine *eng_ = nullptr;
   FilmPool films_;
   std::deque<SchedJob> pending_;
-  std::map<uint64_t, SchedJob> active_;  // by engine sequence (admission order)
+  std::map<uint64_t, SchedJob> jobs_;  // admitted, by ticket (arrival order)
+  std::map<uint64_t, Slice> slices_;   // open slices by engine sequence
+  uint64_t open_items_ = 0;            // unfinished samples of the open slices
   std::vector<hipEvent_t> free_events_;
   int snap_ = 0;
   bool snap_pending_[2] = {false, false};

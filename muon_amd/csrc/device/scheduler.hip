@@ -97,7 +97,7 @@ int Scheduler::Create(muonx_ctx *ctx, const muonx_tuning &tuning, const muonx_en
 
 Scheduler::~Scheduler() {
   if (eng_) (void)hipStreamSynchronize(eng_->stream());
-  for (auto &kv : active_)
+  for (auto &kv : jobs_)
     if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
   for (auto ev : free_events_) (void)hipEventDestroy(ev);
   delete eng_;
@@ -159,7 +159,7 @@ int Scheduler::Submit(const muonx_job &job, uint64_t ticket, uint64_t submit_ns)
   j.ticket = ticket;
   j.spec = job;
   j.film_bytes = FilmBytes(job.params);
-  j.nitems = uint64_t(job.params.width) * uint64_t(job.params.height) * uint64_t(job.params.spp);
+  j.npix = uint32_t(job.params.width) * uint32_t(job.params.height);
   j.state = kPending;
   j.result.status = MUONX_PENDING;
   j.result.submit_ns = submit_ns;
@@ -167,66 +167,102 @@ int Scheduler::Submit(const muonx_job &job, uint64_t ticket, uint64_t submit_ns)
   return MUONX_OK;
 }
 
-int Scheduler::AdmitPending() {
-  while (!pending_.empty()) {
-    SchedJob &j = pending_.front();
-    if (eng_->seq_hi() - eng_->seq_lo() >= eng_->max_jobs()) break;
-    uint64_t off = 0;
-    if (!films_.Alloc(j.film_bytes, &off)) break;  // admission by film memory, in arrival order
-    uint32_t slot = 0;
-    int rc = eng_->AddScene(j.spec.scene, &slot);
-    if (rc == MUONX_OK) {
-      JobSpec js;
-      js.scene_slot = slot;
-      js.params = j.spec.params;
-      js.key = j.spec.key;
-      js.film = films_.at(off);
-      rc = eng_->Admit(js, &j.seq);
+int Scheduler::AdmitSlice(SchedJob &j) {
+  const uint64_t slice_items = std::max<uint64_t>(cfg_.path_slots / 4, 1);
+  const uint32_t remaining = uint32_t(j.spec.params.spp) - j.next_sample;
+  const uint32_t n = uint32_t(std::min<uint64_t>(remaining, std::max<uint64_t>(1, slice_items / j.npix)));
+  JobSpec js;
+  js.scene_slot = j.scene_slot;
+  js.params = j.spec.params;
+  js.key = j.spec.key;
+  js.film = films_.at(j.film_off);
+  js.sample_begin = j.next_sample;
+  js.sample_count = n;
+  js.clear_film = j.next_sample == 0;
+  uint64_t seq = 0;
+  const int rc = eng_->Admit(js, &seq);
+  if (rc != MUONX_OK) return rc;
+  Slice sl;
+  sl.ticket = j.ticket;
+  sl.nitems = uint64_t(j.npix) * n;
+  sl.admit_snap = snap_count_;
+  if (slices_.empty()) progress_iter_ = iterations_;
+  slices_.emplace(seq, sl);
+  open_items_ += sl.nitems;
+  j.next_sample += n;
+  ++j.open_slices;
+  return MUONX_OK;
+}
+
+int Scheduler::AdmitWork() {
+  const uint64_t backlog = 3 * uint64_t(cfg_.path_slots) / 2;
+  while (open_items_ < backlog && eng_->seq_hi() - eng_->seq_lo() < eng_->max_jobs()) {
+    // the oldest waiting job, once its film fits (admission by film memory,
+    // in arrival order)
+    if (!pending_.empty()) {
+      SchedJob &p = pending_.front();
+      uint64_t off = 0;
+      if (films_.Alloc(p.film_bytes, &off)) {
+        uint32_t slot = 0;
+        int rc = eng_->AddScene(p.spec.scene, &slot);
+        if (rc != MUONX_OK) {
+          films_.Free(off, p.film_bytes);
+          return rc;
+        }
+        p.film_off = off;
+        p.scene_slot = slot;
+        p.state = kActive;
+        p.result.admit_ns = NowNs();
+        SchedJob &j = jobs_.emplace(p.ticket, p).first->second;
+        pending_.pop_front();
+        if ((rc = AdmitSlice(j)) != MUONX_OK) return rc;
+        continue;
+      }
     }
-    if (rc != MUONX_OK) {
-      films_.Free(off, j.film_bytes);
-      return rc;
+    // otherwise the next slice of the oldest running job that has samples left
+    SchedJob *next = nullptr;
+    for (auto &kv : jobs_) {
+      SchedJob &j = kv.second;
+      if (j.state == kActive && j.next_sample < uint32_t(j.spec.params.spp)) {
+        next = &j;
+        break;
+      }
     }
-    j.film_off = off;
-    j.scene_slot = slot;
-    j.state = kActive;
-    j.admit_snap = snap_count_;
-    if (active_.empty()) progress_iter_ = iterations_;
-    j.result.admit_ns = NowNs();
-    active_.emplace(j.seq, j);
-    pending_.pop_front();
+    if (!next) break;
+    const int rc = AdmitSlice(*next);
+    if (rc != MUONX_OK) return rc;
   }
   return MUONX_OK;
 }
 
-void Scheduler::AdvanceWindow() {
-  uint64_t lo = eng_->seq_hi();
-  for (const auto &kv : active_) {
-    if (kv.second.state == kActive) {
-      lo = kv.first;
-      break;
-    }
-  }
-  eng_->Retire(lo);
-}
+void Scheduler::AdvanceWindow() { eng_->Retire(slices_.empty() ? eng_->seq_hi() : slices_.begin()->first); }
 
 int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
   (void)finished;
   const unsigned long long *done = eng_->done_host(b);
-  bool any_active = false;
-  for (auto &kv : active_) {
-    SchedJob &j = kv.second;
-    if (j.state != kActive) continue;
-    any_active = true;
-    if (j.admit_snap > snap_id_[b]) continue;  // snapshot older than the job's admission
-    const uint64_t d = done[j.seq % eng_->max_jobs()];
-    if (d > j.seen_done) {
-      j.seen_done = d;
+  if (slices_.empty()) progress_iter_ = iterations_;
+  for (auto it = slices_.begin(); it != slices_.end();) {
+    Slice &sl = it->second;
+    if (sl.admit_snap > snap_id_[b]) {  // snapshot older than the slice's admission
+      ++it;
+      continue;
+    }
+    const uint64_t d = done[it->first % eng_->max_jobs()];
+    if (d > sl.seen_done) {
+      open_items_ -= std::min(d, sl.nitems) - std::min(sl.seen_done, sl.nitems);
+      sl.seen_done = d;
       progress_iter_ = iterations_;
     }
-    if (d < j.nitems) continue;
-    const uint32_t npix = uint32_t(j.spec.params.width) * uint32_t(j.spec.params.height);
-    int rc = eng_->ResolveFilm(films_.at(j.film_off), npix, j.spec.accum_rgb);
+    if (d < sl.nitems) {
+      ++it;
+      continue;
+    }
+    SchedJob &j = jobs_.at(sl.ticket);
+    it = slices_.erase(it);
+    --j.open_slices;
+    if (j.open_slices != 0 || j.next_sample < uint32_t(j.spec.params.spp)) continue;
+    // every sample of the job is in its film
+    int rc = eng_->ResolveFilm(films_.at(j.film_off), j.npix, j.spec.accum_rgb);
     if (rc != MUONX_OK) return rc;
     if (free_events_.empty()) {
       hipEvent_t ev;
@@ -242,7 +278,6 @@ int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
   }
   // Every generated path retires within kMaxBounces + 2 iterations, so a
   // long stretch without a finished sample means the engine lost paths.
-  if (!any_active) progress_iter_ = iterations_;
   if (iterations_ - progress_iter_ > 2 * uint64_t(kMaxBounces) + 64) {
     SetError("engine: no sample finished in " + std::to_string(iterations_ - progress_iter_) + " iterations");
     return MUONX_ERR_STATE;
@@ -256,7 +291,7 @@ int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
 }
 
 int Scheduler::FinishResolves(std::vector<SchedJob> *finished, bool wait_all) {
-  for (auto it = active_.begin(); it != active_.end();) {
+  for (auto it = jobs_.begin(); it != jobs_.end();) {
     SchedJob &j = it->second;
     if (j.state == kResolving) {
       const hipError_t q = wait_all ? hipEventSynchronize(j.ev) : hipEventQuery(j.ev);
@@ -267,7 +302,7 @@ int Scheduler::FinishResolves(std::vector<SchedJob> *finished, bool wait_all) {
         free_events_.push_back(j.ev);
         j.ev = nullptr;
         finished->push_back(j);
-        it = active_.erase(it);
+        it = jobs_.erase(it);
         continue;
       }
       if (q == hipErrorNotReady) {
@@ -283,11 +318,9 @@ int Scheduler::FinishResolves(std::vector<SchedJob> *finished, bool wait_all) {
 }
 
 int Scheduler::Step(std::vector<SchedJob> *finished) {
-  int rc = AdmitPending();
+  int rc = AdmitWork();
   if (rc != MUONX_OK) return rc;
-  bool running = false;
-  for (const auto &kv : active_) running = running || kv.second.state == kActive;
-  if (running) {
+  if (!slices_.empty()) {
     for (uint32_t i = 0; i < cfg_.iterations_per_poll; ++i) {
       if ((rc = eng_->Iterate()) != MUONX_OK) return rc;
       ++iterations_;
@@ -313,9 +346,7 @@ int Scheduler::Step(std::vector<SchedJob> *finished) {
       if ((rc = Harvest(b, finished)) != MUONX_OK) return rc;
     }
   }
-  bool still_running = false;
-  for (const auto &kv : active_) still_running = still_running || kv.second.state == kActive;
-  return FinishResolves(finished, !still_running && pending_.empty());
+  return FinishResolves(finished, slices_.empty() && pending_.empty());
 }
 
 }  // namespace muonx
