@@ -105,6 +105,8 @@ def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 10
     tb = time.perf_counter() - t0
     rays = st.primary_rays + st.secondary_rays + st.shadow_rays
     log_fn(f"bench: batched {tb:.2f} s; sequential plain renders")
+    render_device(jobs[0][0], jobs[0][1])  # the plain path's engine
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     seq = []
     for ds, p, _ in jobs:
@@ -326,9 +328,14 @@ def main():
         torch.cuda.synchronize()
         rs = time.perf_counter() - t0
         rays_total = rst.primary_rays + rst.secondary_rays + rst.shadow_rays
+        traced = rays_total - rst.culled_rays
         return {"seconds": round(rs, 4), "images_per_s": round(1.0 / rs, 3),
                 "mrays_per_s": round(rays_total / rs / 1e6, 1), "primary_rays": rst.primary_rays,
                 "secondary_rays": rst.secondary_rays, "shadow_rays": rst.shadow_rays,
+                # MIS rays whose direction misses every emitter are settled
+                # without a traversal; they stay in secondary_rays (the
+                # reference traces them) and are excluded here
+                "culled_mis_rays": rst.culled_rays, "traced_mrays_per_s": round(traced / rs / 1e6, 1),
                 "mean_radiance": float(accum.mean().item()) / params.spp}
 
     render = None

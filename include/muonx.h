@@ -132,7 +132,10 @@ typedef struct muonx_trace_stats {
   uint64_t primary_rays;
   uint64_t secondary_rays;
   uint64_t shadow_rays;
-  uint64_t render_ns;  /* device time of the render, ns */
+  uint64_t render_ns;   /* device time of the render, ns */
+  uint64_t culled_rays; /* of secondary_rays, MIS rays resolved without a BVH traversal: their
+                           direction misses every emitting primitive, so their closest hit
+                           cannot be a light (path tracer; 0 for the other integrators) */
 } muonx_trace_stats;
 
 typedef struct muonx_scene_info {

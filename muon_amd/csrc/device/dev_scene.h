@@ -55,6 +55,7 @@ constexpr uint32_t kCountMask = 0x1fffffffu;
 constexpr uint32_t kLeafCountShift = 24;
 constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
 constexpr uint32_t kLeafMaxCount = 128;
+constexpr uint32_t kMaxLightPrims = 16;
 constexpr uint32_t kSphereBit = 0x80000000u;
 
 // Material record, 16 floats: ambient3 diffuse3 specular3 emission3
@@ -84,6 +85,12 @@ struct DevSceneView {
   uint32_t num_wnodes;
   uint32_t wstack;          // bound on stack entries of a wide traversal
   uint32_t leaf_counts;     // 1: wide leaf references carry their primitive count
+  // Emitting primitives (prim_light >= 0), leaf order, when there are at
+  // most kMaxLightPrims of them (cull_mis = 1): a MIS ray that hits none of
+  // them needs no traversal.
+  const uint32_t *light_prims;
+  uint32_t num_light_prims;
+  uint32_t cull_mis;
 };
 
 }  // namespace muonx

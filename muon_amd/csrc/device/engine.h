@@ -41,6 +41,11 @@ constexpr int kShadeWaves = kShadeBlock / 64;
 constexpr int kMaxBounces = 1024;  // hard cap for max_depth = -1 without Russian roulette
 constexpr double kFilmScale = 16777216.0;  // fixed-point units per radiance unit
 
+// Ray counters of the engine: primary, secondary (continuation and MIS rays,
+// as the reference counts them), shadow, and the MIS rays among the
+// secondary ones resolved without a traversal.
+constexpr int kRayCounters = 4;
+
 struct JobDev {
   CamParams cam;
   uint32_t scene;  // scene table slot
@@ -64,6 +69,12 @@ struct alignas(256) HotCounter {
   uint32_t pad[63];
 };
 
+struct alignas(256) HotCounter64 {
+  unsigned long long v;
+  unsigned long long pad[31];
+};
+constexpr int kRaySlots = 16;  // partial ray counts per counter, blocks spread over them
+
 struct EngCounters {
   HotCounter qcount[2];  // path queues
   HotCounter mis_count;
@@ -74,7 +85,11 @@ struct EngCounters {
   unsigned long long cursor;          // next work item to generate
   unsigned long long total_items;     // admitted work (host-written)
   unsigned long long seq_lo, seq_hi;  // admitted jobs (host-written)
-  unsigned long long rays[3];         // primary, secondary, shadow
+  unsigned long long rays[kRayCounters];  // primary, secondary, shadow, culled MIS
+  // This iteration's counts by kernel block (block % kRaySlots), folded into
+  // `rays` by the plan kernel: one counter updated by every wavefront of a
+  // launch would serialise its atomics.
+  HotCounter64 ray_slots[kRayCounters][kRaySlots];
 };
 
 // Per-path state, struct of arrays over queue positions.
@@ -151,7 +166,7 @@ class Engine {
   // Converts a job's fixed-point film to float sums on the engine stream.
   int ResolveFilm(const unsigned long long *film, uint32_t npix, float *out);
   // Ray counters (primary, secondary, shadow) since the last reset; waits.
-  int RayCounts(unsigned long long out[3]);
+  int RayCounts(unsigned long long out[kRayCounters]);
   int ResetRayCounts();
 
   hipStream_t stream() const { return stream_; }

@@ -11,11 +11,17 @@
 
 namespace muonx {
 
-
 __device__ __forceinline__ unsigned long long to_fixed(float v) {
   if (!(v > 0.0f)) return 0ull;  // zero, negative or NaN
   const double x = double(v) * kFilmScale + 0.5;
   return x < 4.0e18 ? (unsigned long long)x : 4000000000000000000ull;
+}
+
+// Adds a wavefront's count to this block's slot of ray counter `id`.
+__device__ __forceinline__ void ray_count(EngCounters *c, int id, uint32_t n) {
+  unsigned long long v = n;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if (__lane_id() == 0 && v) atomicAdd(&c->ray_slots[id][blockIdx.x % kRaySlots].v, v);
 }
 
 // ---------------------------------------------------------------- extend
@@ -49,8 +55,8 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
       a.hit_prim[i] = hit;
     }
   }
-  wave_count(prim_rays, &a.c->rays[0]);
-  wave_count(sec_rays, &a.c->rays[1]);
+  ray_count(a.c, 0, prim_rays);
+  ray_count(a.c, 1, sec_rays);
 }
 
 // ---------------------------------------------------------------- shade
@@ -195,6 +201,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_eng_shade_light(EngArgs a, int 
 __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
     k_eng_shade_bsdf(EngArgs a, int cur) {
   __shared__ uint32_t wsum[kShadeWaves + 1];
+  uint32_t culled_n = 0;
   const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
   float4 *qn = a.q_ray[cur ^ 1];
@@ -202,9 +209,10 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
   const PathState &pn = a.st[cur ^ 1];
   for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
-    bool cont = false, want_mis = false;
+    bool cont = false, want_mis = false, culled = false;
     uint32_t js = 0, pixel = 0, sample = 0, ctr = 0;
     int depth = 0;
+    float mis_tmax = __builtin_inff();
     V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0}, T{0, 0, 0};
     if (i < n && a.hit_prim[i] >= 0) {
       js = ps.job[i];
@@ -234,6 +242,28 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
           a.m_fac[3 * i + 2] = f.z;
           want_mis = true;
           mis_dir = dir;
+          if (s.cull_mis) {
+            // The term is nonzero only when the ray's closest hit is an
+            // emitting primitive. Test the few emitters with the traversal's
+            // own primitive test: a ray that hits none is settled here (no
+            // light hit, m_hit stays -1); otherwise the traversal only needs
+            // to look up to the nearest emitter hit.
+            const TraceRay mr = make_ray(shift, dir);
+            ObjCache oc;
+            oc.tid = -1;
+            float dmin = __builtin_inff();
+            for (uint32_t k = 0; k < s.num_light_prims; ++k) {
+              float dist;
+              uint32_t flags;
+              if (prim_hit(s, s.light_prims[k], mr, oc, dist, flags) && dist > 0.0f && dist < dmin) dmin = dist;
+            }
+            if (dmin == __builtin_inff()) {
+              want_mis = false;
+              culled = true;
+            } else {
+              mis_tmax = dmin * kWidenHi;
+            }
+          }
         }
       }
       // ShadeIndirect (muon/integration.cc:294-328)
@@ -273,9 +303,13 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
     const uint32_t mi = block_append<kShadeWaves>(want_mis, &a.c->mis_count.v, wsum);
     if (want_mis) {
       a.m_ray[2 * mi] = make_float4(shift.x, shift.y, shift.z, __uint_as_float(i));
-      a.m_ray[2 * mi + 1] = make_float4(mis_dir.x, mis_dir.y, mis_dir.z, 0.0f);
+      a.m_ray[2 * mi + 1] = make_float4(mis_dir.x, mis_dir.y, mis_dir.z, mis_tmax);
     }
+    culled_n += culled ? 1u : 0u;
   }
+  // settled MIS rays count as traced secondary rays (the reference traces them)
+  ray_count(a.c, 1, culled_n);
+  ray_count(a.c, 3, culled_n);
 }
 
 // ---------------------------------------------------------------- shadow (any hit)
@@ -302,7 +336,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs s
       ++traced;
     }
   }
-  wave_count(traced, &a.c->rays[2]);
+  ray_count(a.c, 2, traced);
 }
 
 // ---------------------------------------------------------------- MIS extend
@@ -319,14 +353,14 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, 
       const uint32_t i = __float_as_uint(mo.w);
       const DevSceneView s = a.scenes[a.jobs[ps.job[i]].scene];
       const TraceRay r = make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z});
-      float best = __builtin_inff();
+      float best = md.w;  // the nearest emitter hit, or +inf
       TraceCounters c;
       auto st = make_stack<KIND>(lds, kEngBlock, sa);
       a.m_hit[i] = trace_fast<WIDTH, false, false, WW>(s, r, best, st, c);
       ++traced;
     }
   }
-  wave_count(traced, &a.c->rays[1]);
+  ray_count(a.c, 1, traced);
 }
 
 // ---------------------------------------------------------------- per-lane fetch variants
@@ -378,8 +412,8 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
       have = false;
     }
   }
-  wave_count(prim_rays, &a.c->rays[0]);
-  wave_count(sec_rays, &a.c->rays[1]);
+  ray_count(a.c, 0, prim_rays);
+  ray_count(a.c, 1, sec_rays);
 }
 
 template <int KIND, int WIDTH>
@@ -420,7 +454,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
       have = false;
     }
   }
-  wave_count(traced, &a.c->rays[2]);
+  ray_count(a.c, 2, traced);
 }
 
 template <int KIND, int WIDTH>
@@ -445,7 +479,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
           const float4 mo = a.m_ray[2 * k], md = a.m_ray[2 * k + 1];
           i = __float_as_uint(mo.w);
           sv = a.scenes[a.jobs[ps.job[i]].scene];
-          lt.start(make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z}), __builtin_inff(), st);
+          lt.start(make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z}), md.w, st);
           have = true;
           ++traced;
         } else {
@@ -459,7 +493,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
       have = false;
     }
   }
-  wave_count(traced, &a.c->rays[1]);
+  ray_count(a.c, 1, traced);
 }
 
 // ---------------------------------------------------------------- retire
@@ -557,6 +591,14 @@ __global__ void k_eng_plan(EngArgs a, int cur) {
   c->mis_count.v = 0;
   c->shadow_count.v = 0;
   c->claim[0].v = c->claim[1].v = c->claim[2].v = 0;
+  for (int id = 0; id < kRayCounters; ++id) {
+    unsigned long long sum = 0;
+    for (int k = 0; k < kRaySlots; ++k) {
+      sum += c->ray_slots[id][k].v;
+      c->ray_slots[id][k].v = 0;
+    }
+    c->rays[id] += sum;
+  }
   const uint32_t next = c->qcount[cur ^ 1].v;
   const unsigned long long avail = c->total_items > c->cursor ? c->total_items - c->cursor : 0ull;
   const uint32_t room = a.capacity - next;
@@ -743,8 +785,8 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
   if (hipHostMalloc(reinterpret_cast<void **>(&e->h_jobs_), sizeof(JobDev) * max_jobs) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&e->h_first_), sizeof(unsigned long long) * max_jobs) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&e->h_window_), sizeof(unsigned long long) * 3 * 4) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&e->h_done_[0]), sizeof(unsigned long long) * (max_jobs + 3)) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&e->h_done_[1]), sizeof(unsigned long long) * (max_jobs + 3)) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void **>(&e->h_done_[0]), sizeof(unsigned long long) * (max_jobs + kRayCounters)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->h_done_[1]), sizeof(unsigned long long) * (max_jobs + kRayCounters)) != hipSuccess) {
     SetError("engine: pinned host allocation failed");
     return fail(MUONX_ERR_NOMEM);
   }
@@ -974,7 +1016,7 @@ int Engine::Iterate() {
 int Engine::SnapshotDone(int b) {
   MX_HIP(hipMemcpyAsync(h_done_[b], done_.ptr, sizeof(unsigned long long) * max_jobs_, hipMemcpyDeviceToHost,
                         stream_));
-  MX_HIP(hipMemcpyAsync(h_done_[b] + max_jobs_, &a_.c->rays[0], 3 * sizeof(unsigned long long),
+  MX_HIP(hipMemcpyAsync(h_done_[b] + max_jobs_, &a_.c->rays[0], kRayCounters * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, stream_));
   MX_HIP(hipEventRecord(done_ev_[b], stream_));
   return MUONX_OK;
@@ -994,12 +1036,13 @@ int Engine::ResolveFilm(const unsigned long long *film, uint32_t npix, float *ou
 }
 
 int Engine::ResetRayCounts() {
-  MX_HIP(hipMemsetAsync(&a_.c->rays[0], 0, 3 * sizeof(unsigned long long), stream_));
+  MX_HIP(hipMemsetAsync(&a_.c->rays[0], 0, kRayCounters * sizeof(unsigned long long), stream_));
   return MUONX_OK;
 }
 
-int Engine::RayCounts(unsigned long long out[3]) {
-  MX_HIP(hipMemcpyAsync(out, &a_.c->rays[0], 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+int Engine::RayCounts(unsigned long long out[kRayCounters]) {
+  MX_HIP(hipMemcpyAsync(out, &a_.c->rays[0], kRayCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        stream_));
   MX_HIP(hipStreamSynchronize(stream_));
   return MUONX_OK;
 }

@@ -34,14 +34,15 @@ bool SameConfig(const muonx_engine_config &a, const muonx_engine_config &b) {
 }
 
 // The context's scheduler for `jobs`: with `want`, one built from exactly
-// that configuration; otherwise any that can run every job. Called with the
-// context lock held.
+// that configuration; otherwise one built for the plain path (capacity from
+// the tuning, not a caller's batch configuration) that can run every job.
+// Called with the context lock held.
 int ContextScheduler(muonx_ctx *ctx, const muonx_engine_config *want, const muonx_job *jobs, uint32_t n,
                      Scheduler **out) {
   RenderWorkspace *ws = GetWorkspace(ctx);
   Scheduler *s = ws->sched;
   bool ok = s && SameTuning(ws->sched_tuning, ctx->tuning);
-  if (ok && want) ok = SameConfig(ws->sched_want, *want);
+  if (ok) ok = want ? SameConfig(ws->sched_want, *want) : ws->sched_plain;
   for (uint32_t i = 0; ok && i < n; ++i) ok = s->Validate(jobs[i]) == MUONX_OK;
   if (ok) {
     *out = s;
@@ -72,6 +73,7 @@ int ContextScheduler(muonx_ctx *ctx, const muonx_engine_config *want, const muon
   if (rc != MUONX_OK) return rc;
   ws->sched_tuning = ctx->tuning;
   ws->sched_want = want ? *want : muonx_engine_config{};
+  ws->sched_plain = want == nullptr;
   for (uint32_t i = 0; i < n; ++i)
     if ((rc = ws->sched->Validate(jobs[i])) != MUONX_OK) return rc;
   *out = ws->sched;
@@ -79,7 +81,8 @@ int ContextScheduler(muonx_ctx *ctx, const muonx_engine_config *want, const muon
 }
 
 // Runs `jobs` to completion on `s`; results[i] is job i's.
-int RunInline(Scheduler *s, const muonx_job *jobs, uint32_t n, muonx_job_result *results, unsigned long long rays[3]) {
+int RunInline(Scheduler *s, const muonx_job *jobs, uint32_t n, muonx_job_result *results,
+              unsigned long long rays[kRayCounters]) {
   int rc = s->engine()->ResetRayCounts();
   if (rc != MUONX_OK) return rc;
   const uint64_t t0 = NowNs();
@@ -91,14 +94,14 @@ int RunInline(Scheduler *s, const muonx_job *jobs, uint32_t n, muonx_job_result 
     if ((rc = s->Step(&finished)) != MUONX_OK) return rc;
   if (results)
     for (const SchedJob &j : finished) results[j.ticket] = j.result;
-  if (rays) std::memcpy(rays, s->rays(), 3 * sizeof(unsigned long long));
+  if (rays) std::memcpy(rays, s->rays(), kRayCounters * sizeof(unsigned long long));
   return MUONX_OK;
 }
 
 }  // namespace
 
 int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *accum, hipStream_t stream,
-                     unsigned long long rays[3]) {
+                     unsigned long long rays[kRayCounters]) {
   muonx_ctx *ctx = scene->ctx;
   muonx_job job{};
   job.scene = scene;
@@ -139,13 +142,14 @@ extern "C" int muonx_render_batch(muonx_ctx *ctx, const muonx_job *jobs, uint32_
   int rc = ContextScheduler(ctx, cfg, jobs, n, &s);
   if (rc != MUONX_OK) return rc;
   const uint64_t t0 = NowNs();
-  unsigned long long rays[3] = {0, 0, 0};
+  unsigned long long rays[kRayCounters] = {0, 0, 0, 0};
   rc = RunInline(s, jobs, n, results, rays);
   if (rc != MUONX_OK) return rc;
   if (stats) {
     stats->primary_rays = rays[0];
     stats->secondary_rays = rays[1];
     stats->shadow_rays = rays[2];
+    stats->culled_rays = rays[3];
     stats->render_ns = NowNs() - t0;
   }
   return MUONX_OK;

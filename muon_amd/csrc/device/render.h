@@ -64,6 +64,7 @@ struct RenderWorkspace {
   Scheduler *sched = nullptr;
   muonx_tuning sched_tuning{};
   muonx_engine_config sched_want{};
+  bool sched_plain = false;  // built by the plain path (muonx_render), not for a batch configuration
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
@@ -74,7 +75,7 @@ CamParams MakeCam(const muonx_render_params &p);
 // `stream` (later work on `stream` waits for the film). rays: primary,
 // secondary, shadow.
 int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *accum, hipStream_t stream,
-                     unsigned long long rays[3]);
+                     unsigned long long rays[4]);  // primary, secondary, shadow, culled MIS
 
 }  // namespace muonx
 

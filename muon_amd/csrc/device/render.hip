@@ -303,7 +303,7 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
   int rc;
   unsigned long long *ctr = ws->counters.as<unsigned long long>();
   bool host_rays = false;
-  unsigned long long rays[3] = {0, 0, 0};
+  unsigned long long rays[4] = {0, 0, 0, 0};
   switch (p.integrator) {
     case MUONX_INTEGRATOR_NORMALS:
     case MUONX_INTEGRATOR_ALBEDO:
@@ -338,6 +338,7 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
     stats->secondary_rays = host_rays ? rays[1] : h[kCtrSecondary];
     stats->shadow_rays = host_rays ? rays[2] : h[kCtrShadow];
     stats->render_ns = uint64_t(double(ms) * 1e6);
+    stats->culled_rays = host_rays ? rays[3] : 0;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

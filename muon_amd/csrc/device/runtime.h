@@ -83,6 +83,7 @@ struct muonx_scene {
   uint64_t id = 0;
   std::atomic<int> inflight{0};  // server jobs that use the scene
   muonx::DevBuf nodes, wnodes, qnodes, prims, xforms, prim_id, prim_mat, prim_light, prim_vn, prim_use_vn, materials, lights;
+  muonx::DevBuf light_prims;  // emitting primitives (leaf order) when few
   muonx::DevSceneView view{};
   uint32_t bvh_depth = 0;
   uint64_t device_bytes = 0;

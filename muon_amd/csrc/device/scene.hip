@@ -454,12 +454,18 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   uint64_t total = 0;
   int rc = MUONX_OK;
   const std::vector<uint4> qnodes = QuantizeWide(wnodes);
+  std::vector<uint32_t> lprims;
+  for (uint32_t k = 0; k < np; ++k)
+    if (plight[k] >= 0) lprims.push_back(k);
+  const bool cull_mis = lprims.size() <= kMaxLightPrims;
+  if (!cull_mis || lprims.empty()) lprims.assign(1, 0u);  // keep the buffer non-empty
   if ((rc = Upload(s->nodes, nodes, total)) || (rc = Upload(s->wnodes, wnodes, total)) ||
       (rc = Upload(s->qnodes, qnodes, total)) ||
       (rc = Upload(s->prims, prims, total)) ||
       (rc = Upload(s->xforms, xf, total)) || (rc = Upload(s->prim_id, pid, total)) ||
       (rc = Upload(s->prim_mat, pmat, total)) || (rc = Upload(s->prim_light, plight, total)) ||
-      (rc = Upload(s->materials, mats, total)) || (rc = Upload(s->lights, lights, total))) {
+      (rc = Upload(s->materials, mats, total)) || (rc = Upload(s->lights, lights, total)) ||
+      (rc = Upload(s->light_prims, lprims, total))) {
     delete s;
     return rc;
   }
@@ -476,6 +482,13 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   v.num_wnodes = uint32_t(wnodes.size() / 8);
   v.wstack = wstack;
   v.leaf_counts = leaf_counts ? 1u : 0u;
+  v.light_prims = s->light_prims.as<uint32_t>();
+  v.cull_mis = cull_mis ? 1u : 0u;
+  {
+    uint32_t ne = 0;
+    for (uint32_t k = 0; k < np; ++k) ne += plight[k] >= 0 ? 1u : 0u;
+    v.num_light_prims = cull_mis ? ne : 0u;
+  }
   v.prims = s->prims.as<float4>();
   v.xforms = s->xforms.as<float>();
   v.prim_id = s->prim_id.as<int32_t>();

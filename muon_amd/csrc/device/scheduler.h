@@ -119,7 +119,7 @@ class Scheduler {
   uint64_t snap_count_ = 0;
   uint64_t iterations_ = 0;
   uint64_t progress_iter_ = 0;  // iteration count when a sample last finished
-  unsigned long long rays_[3] = {0, 0, 0};
+  unsigned long long rays_[kRayCounters] = {0, 0, 0, 0};
 };
 
 uint64_t NowNs();

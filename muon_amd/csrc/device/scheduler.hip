@@ -283,9 +283,7 @@ int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
     return MUONX_ERR_STATE;
   }
   const unsigned long long *r = eng_->rays_host(b);
-  rays_[0] = r[0];
-  rays_[1] = r[1];
-  rays_[2] = r[2];
+  for (int k = 0; k < kRayCounters; ++k) rays_[k] = r[k];
   AdvanceWindow();
   return MUONX_OK;
 }

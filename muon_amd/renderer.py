@@ -37,6 +37,7 @@ class TraceStats:
     shadow_rays: int = 0
     render_seconds: float = 0.0
     build_seconds: float = 0.0
+    culled_rays: int = 0  # of secondary_rays: MIS rays settled without a traversal
 
     def __str__(self):
         lines = [
@@ -86,7 +87,8 @@ def render_device(dev_scene, params, accum=None):
     st = _lib.TraceStatsC()
     s = torch.cuda.current_stream(accum.device).cuda_stream
     check(lib.muonx_render(dev_scene.h, C.byref(params), accum.data_ptr(), C.byref(st), s))
-    stats = TraceStats(st.primary_rays, st.secondary_rays, st.shadow_rays, st.render_ns * 1e-9)
+    stats = TraceStats(st.primary_rays, st.secondary_rays, st.shadow_rays, st.render_ns * 1e-9,
+                       culled_rays=st.culled_rays)
     return accum, stats
 
 

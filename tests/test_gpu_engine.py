@@ -141,17 +141,23 @@ def test_server_cancel(ctx, resident):
     MUONX_ERR_CANCELLED); cancelling A, already rendering, changes nothing;
     C runs after A. A and C equal their renders alone."""
     hs, ds = resident["ico"]
-    p = params_of(hs, width=512, height=512, spp=64, max_depth=8)
+    # A renders for tens of milliseconds, so B is still waiting when cancelled
+    p = params_of(hs, width=512, height=512, spp=128, max_depth=8)
     q = params_of(hs, width=512, height=512, spp=4, max_depth=8, seed=3)
     ref_a, ref_c = alone(ds, p), alone(ds, q)
     cfg = engine_config(max_jobs=4, film_budget_bytes=512 * 512 * 24 + 512 * 512 * 12)
+    sentinel = torch.full((512, 512, 3), 7.0, dtype=torch.float32, device="cuda:0")
+    out_c = torch.empty((512, 512, 3), dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
     with Server(ctx, cfg) as srv:
         ta = srv.submit(ds, p)
-        sentinel = torch.full((512, 512, 3), 7.0, dtype=torch.float32, device="cuda:0")
+        t_end = time.monotonic() + 30.0
+        while srv.stats()["in_flight"] == 0 and time.monotonic() < t_end:  # until A is admitted
+            time.sleep(0.0005)
         tb = srv.submit(ds, q, accum=sentinel)
-        tc = srv.submit(ds, q)
+        tc = srv.submit(ds, q, accum=out_c)
         srv.cancel(tb)
-        srv.cancel(ta)  # rendering (or about to): completes normally
+        srv.cancel(ta)  # rendering: completes normally
         acc_b, res_b = srv.wait(tb, timeout_ms=60000)
         assert acc_b is None and res_b.status == _lib.MUONX_ERR_CANCELLED
         acc_a, res_a = srv.wait(ta, timeout_ms=60000)
