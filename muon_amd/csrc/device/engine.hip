@@ -65,7 +65,8 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
 //   shade_light  hit record, emission, NEE shadow rays (ShadeDirectNEE)
 //   shade_bsdf   MIS BRDF ray (ShadeDirectImportanceSampling) and the
 //                continuation with Russian roulette (ShadeIndirect)
-__global__ void __launch_bounds__(kShadeBlock) k_eng_shade_light(EngArgs a, int cur) {
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    k_eng_shade_light(EngArgs a, int cur) {
   __shared__ uint32_t wsum[kShadeWaves + 1];
   const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_eng_shade_light(EngArgs a, int 
   }
 }
 
-__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
     k_eng_shade_bsdf(EngArgs a, int cur) {
   __shared__ uint32_t wsum[kShadeWaves + 1];
   uint32_t culled_n = 0;
