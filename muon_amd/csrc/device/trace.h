@@ -32,13 +32,12 @@ struct TraceRay {
   uint32_t dneg;  // bit a set when d[a] < 0
 };
 
-__device__ __forceinline__ float safe_inv(float d) {
-  // Axes with a zero direction component are skipped by the reference; a
-  // huge reciprocal gives the same outcome for origins inside the slab.
-  float r = 1.0f / d;
-  if (!(fabsf(r) < 1e30f)) r = copysignf(1e30f, d);
-  return r;
-}
+// The reciprocal direction of the fast slab test, infinite for a zero
+// component (the sign follows the zero's sign). The reference skips such an
+// axis (muon/bounds.cc:98-100); here a plane then yields +-inf, or NaN for an
+// origin exactly on the plane, and the slab test's min / max ignore NaN, so
+// the axis only rejects boxes the ray cannot reach.
+__device__ __forceinline__ float safe_inv(float d) { return 1.0f / d; }
 
 __device__ __forceinline__ TraceRay make_ray(V3 o, V3 d) {
   TraceRay r;
@@ -58,11 +57,15 @@ __device__ __forceinline__ float below(float b) { return __uint_as_float(__float
 // from behind, h == 0, is visited and finds nothing).
 __device__ __forceinline__ bool box_hit(const TraceRay &r, float mnx, float mxx, float mny, float mxy, float mnz,
                                         float mxz, float best_m, float &lo) {
-  float ax = (mnx - r.o.x) * r.inv.x, bx = (mxx - r.o.x) * r.inv.x;
-  float ay = (mny - r.o.y) * r.inv.y, by = (mxy - r.o.y) * r.inv.y;
-  float az = (mnz - r.o.z) * r.inv.z, bz = (mxz - r.o.z) * r.inv.z;
-  float l = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
-  float h = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+  // entry (near) and exit (far) plane per axis from the reciprocal's sign;
+  // a NaN distance (origin on a plane of a zero-direction axis) drops out of
+  // the max / min
+  const bool sx = signbit(r.inv.x), sy = signbit(r.inv.y), sz = signbit(r.inv.z);
+  const float ax = ((sx ? mxx : mnx) - r.o.x) * r.inv.x, bx = ((sx ? mnx : mxx) - r.o.x) * r.inv.x;
+  const float ay = ((sy ? mxy : mny) - r.o.y) * r.inv.y, by = ((sy ? mny : mxy) - r.o.y) * r.inv.y;
+  const float az = ((sz ? mxz : mnz) - r.o.z) * r.inv.z, bz = ((sz ? mnz : mxz) - r.o.z) * r.inv.z;
+  float l = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
+  float h = fminf(fminf(bx, by), bz);
   l = l * kWidenLo;
   h = h * kWidenHi;
   lo = l;
@@ -474,8 +477,9 @@ __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay
   constexpr float kInf = __builtin_inff();
   // Per axis, the words holding the planes the ray enters (near) and leaves
   // (far) by: the min planes where the reciprocal direction is positive, the
-  // max planes where it is negative. The same planes the min / max of the
-  // two slab distances pick, chosen once for the four children.
+  // max planes where it is negative, chosen once for the four children. A
+  // NaN distance (zero direction component, origin on the plane) drops out
+  // of the max / min below.
   const uint32_t mx = uint32_t(__float_as_int(r.inv.x) >> 31), my = uint32_t(__float_as_int(r.inv.y) >> 31),
                  mz = uint32_t(__float_as_int(r.inv.z) >> 31);
   const uint32_t nxw = (n1.w & mx) | (n1.x & ~mx), fxw = (n1.x & mx) | (n1.w & ~mx);
