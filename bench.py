@@ -26,7 +26,10 @@ launch configuration for this scene and render (the microbench above runs
 with the library's default configuration):
 * render: one full path-traced render of the same scene (512x512, 64 spp,
   depth 8, BRDF IS + NEE + MIS); render.sphere_field: the 10,000-sphere field
-  with 8 emissive spheres (1024x1024, 32 spp, depth 8, BRDF sampling, NEE off).
+  with 8 emissive spheres (1024x1024, 32 spp, depth 8, BRDF sampling, NEE off);
+  render.cornell_256: the Cornell box at 256x256, 16 spp, depth 5 (the
+  reference's CPU-runnable case), with the CPU restatement's render time of
+  the same image in cpu_baseline.cornell_256_render.
 * batched: 64 render jobs of uneven size (resolution uniform in
   {128^2, 256^2, 512^2, 1024^2}, spp uniform in [4, 64], depth 8, seeded
   camera poses)
@@ -319,6 +322,15 @@ def main():
         cpu = {"value": round(done / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"first {done} of the same microbench rays, closest hit, {threads} threads, {secs:.1f} s"}
         log(f"bench: cpu baseline {cpu['value']} Mrays/s")
+        if not args.no_render:
+            # the reference's own CPU case: the Cornell box at 256x256, 16 spp,
+            # depth 5, rendered by the CPU restatement (render time as the
+            # reference's Stats measures it: tiles only, no parse / build)
+            c0 = orc.load_text(scenes.cornell_text(256, 256, 16, 5, seed=0))
+            _, _, cst, csecs = c0.render(threads)
+            cpu["cornell_256_render"] = {"seconds": round(float(csecs), 3), "threads": threads,
+                                         "reference_rays": int(cst[0] + cst[1])}
+            log(f"bench: cpu cornell 256x256 16spp render {csecs:.2f} s")
 
     def timed_render(scene, params):
         accum, rst = render_device(scene, params)  # warm the workspace
@@ -350,6 +362,16 @@ def main():
         render["sphere_field"] = timed_render(ds2, hs2.params)
         log(f"bench: sphere field {render['sphere_field']}")
         del ds2
+    if not args.no_render:
+        log("bench: Cornell box 256x256 16spp depth 5 (the reference's CPU case)")
+        hs0 = HostScene.from_text(scenes.cornell_text(256, 256, 16, 5, seed=0))
+        ds0 = DeviceScene(ctx, hs0)
+        render["cornell_256"] = timed_render(ds0, hs0.params)
+        if cpu and "cornell_256_render" in cpu:
+            render["cornell_256"]["gpu_over_cpu"] = round(cpu["cornell_256_render"]["seconds"] /
+                                                          render["cornell_256"]["seconds"], 1)
+        log(f"bench: cornell 256 {render['cornell_256']}")
+        del ds0
 
     batched = serving = None
     if not args.no_batch:
