@@ -155,6 +155,10 @@ class Engine {
   void Retire(uint64_t seq_lo) { seq_lo_ = seq_lo; }
   // Launches one iteration (all kernels) on the engine stream.
   int Iterate();
+  // With no path in flight (every admitted sample finished): generates the
+  // first paths of newly admitted work without running the traversal and
+  // shading kernels of an iteration over an empty queue.
+  int Prime();
   // Enqueues a copy of the per-job-slot done counters (index = seq %
   // max_jobs) into host buffer `b` (0 or 1) and marks it with an event.
   int SnapshotDone(int b);

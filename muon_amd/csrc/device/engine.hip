@@ -982,6 +982,16 @@ int Engine::LaunchIteration(int cur) {
   return MUONX_OK;
 }
 
+int Engine::Prime() {
+  int rc = SyncWindow();
+  if (rc != MUONX_OK) return rc;
+  hipLaunchKernelGGL(k_eng_plan, dim3(1), dim3(64), 0, stream_, a_, cur_);
+  hipLaunchKernelGGL(k_eng_fill, dim3(cfg_.grid), dim3(kEngBlock), 0, stream_, a_, cur_);
+  MX_HIP(hipGetLastError());
+  cur_ ^= 1;
+  return MUONX_OK;
+}
+
 int Engine::Iterate() {
   if (!a_.scenes) {
     SetError("engine: no scene registered");

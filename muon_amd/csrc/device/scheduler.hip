@@ -316,8 +316,10 @@ int Scheduler::FinishResolves(std::vector<SchedJob> *finished, bool wait_all) {
 }
 
 int Scheduler::Step(std::vector<SchedJob> *finished) {
+  const bool idle = slices_.empty();  // every admitted sample has finished: no path in flight
   int rc = AdmitWork();
   if (rc != MUONX_OK) return rc;
+  if (idle && !slices_.empty() && (rc = eng_->Prime()) != MUONX_OK) return rc;
   if (!slices_.empty()) {
     for (uint32_t i = 0; i < cfg_.iterations_per_poll; ++i) {
       if ((rc = eng_->Iterate()) != MUONX_OK) return rc;
