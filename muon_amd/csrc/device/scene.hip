@@ -157,18 +157,22 @@ uint32_t FloatBits(float f) {
   return u;
 }
 
-// The device's decode of a quantized plane: origin + q * 2^e, with the
-// multiplication exact and one rounding in the addition.
-float Dequant(float origin, uint32_t q, int e) { return origin + float(q) * std::ldexp(1.0f, e); }
+// A quantized plane origin + q * 2^e as a real number. The double sum is
+// exact: with q < 256, the terms span at most 53 bits whenever the node's
+// extent is nonzero (the extent is then at least one float ulp of the origin,
+// so 2^e >= ulp(origin) / 256), and q is 0 when the extent is 0.
+double PlaneValue(float origin, uint32_t q, int e) { return double(origin) + std::ldexp(double(q), e); }
 
 // Quantizes the 4-wide nodes (8 float4 each) to 64 B nodes (4 uint4 each):
 //   n0 = (origin.xyz, e.x + 127 | (e.y + 127) << 8 | (e.z + 127) << 16 | count << 24)
 //   n1 = (lo.x[4], lo.y[4], lo.z[4], hi.x[4]) as bytes (child k in byte k)
 //   n2 = (hi.y[4], hi.z[4], 0, 0)
 //   n3 = refs[4]
-// with child plane = origin + q * 2^e. Every decoded box contains the child's
-// float box (checked with the decode arithmetic itself), so a traversal over
-// the quantized nodes visits a superset of the boxes.
+// with child plane = origin + q * 2^e. Every quantized box contains the
+// child's float box in exact arithmetic (so also after rounding the plane to
+// float, which is monotonic); the device's slab test is conservative against
+// these real-valued planes, so a traversal over the quantized nodes visits a
+// superset of the boxes.
 std::vector<uint4> QuantizeWide(const std::vector<float4> &w) {
   const size_t nn = w.size() / 8;
   std::vector<uint4> out(nn * 4);
@@ -209,9 +213,9 @@ std::vector<uint4> QuantizeWide(const std::vector<float4> &w) {
           int64_t l = int64_t(std::floor((double(lo[a][k]) - double(mn)) / sc));
           int64_t h = int64_t(std::ceil((double(hi[a][k]) - double(mn)) / sc));
           l = std::max<int64_t>(0, std::min<int64_t>(255, l));
-          while (l > 0 && Dequant(mn, uint32_t(l), e) > lo[a][k]) --l;
+          while (l > 0 && PlaneValue(mn, uint32_t(l), e) > double(lo[a][k])) --l;
           h = std::max<int64_t>(0, h);
-          while (h <= 255 && Dequant(mn, uint32_t(h), e) < hi[a][k]) ++h;
+          while (h <= 255 && PlaneValue(mn, uint32_t(h), e) < double(hi[a][k])) ++h;
           if (h > 255) {
             ok = false;
             break;

@@ -26,6 +26,7 @@ namespace muonx {
 
 constexpr float kWidenLo = 1.0f - 1.0f / 1048576.0f;
 constexpr float kWidenHi = 1.0f + 1.0f / 1048576.0f;
+constexpr float kSlack = 1.0f / 1048576.0f;
 
 struct TraceRay {
   V3 o, d, inv;
@@ -454,15 +455,24 @@ constexpr int kNodesBinary = 2;  // binary child-boxes-in-parent nodes
 constexpr int kNodesWide = 4;    // 4-wide nodes, float child boxes
 constexpr int kNodesWideQ = 5;   // 4-wide nodes, 8-bit quantized child boxes
 
-// origin + q * 2^e: the product is exact (8-bit q, power-of-two scale), so
-// the fused form rounds once exactly like the host's add, in one instruction.
-__device__ __forceinline__ float dequant(float origin, uint32_t word, int k, float scale) {
-  return __builtin_fmaf(float((word >> (8 * k)) & 255u), scale, origin);
-}
+// Byte k of a packed plane word as a float (v_cvt_f32_ubyte<k>).
+__device__ __forceinline__ float plane_q(uint32_t word, int k) { return float((word >> (8 * k)) & 255u); }
 
-// One quantized wide-node step (as wnode_step): decodes each child plane as
-// origin + q * 2^e (exact product, one rounding, as the host checked) and
-// applies the conservative slab test.
+// One quantized wide-node step (as wnode_step). Per axis, the distance to
+// the plane origin + q * 2^e is folded to one fused multiply-add,
+//   t = q * A + B,  A = 2^e * inv (exact),  B = (origin - o) * inv,
+// instead of decoding the plane and then subtracting and multiplying. The
+// fold loses the plane's own rounding, so the host quantizes against the
+// planes' exact real values; the rounding of B (two operations) and of the
+// fma leaves |t' - t| <= ~3 eps |B| + eps |t|. The slack 2^-20 |B| per axis
+// (down on entry planes, up on exit planes) covers the first term and the
+// 2^-20 relative widening the second, so the test stays a superset of the
+// exact one. Reciprocals are clamped to +-2^64: a zero direction component
+// then gives finite plane distances whose sign says whether the origin lies
+// in the slab (an infinite one would make inf - inf and lose the axis), and
+// the per-axis slack keeps a ray that runs in an exit plane inside the box
+// (B is then nonzero unless the child is flat at the node's origin, and a
+// flat child holds only primitives parallel to that ray, which never hit).
 template <bool COUNT, class Stack>
 __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
                                            uint32_t &ref, TraceCounters &cnt) {
@@ -486,13 +496,20 @@ __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay
   const uint32_t nyw = (n2.x & my) | (n1.y & ~my), fyw = (n1.y & my) | (n2.x & ~my);
   const uint32_t nzw = (n2.y & mz) | (n1.z & ~mz), fzw = (n1.z & mz) | (n2.y & ~mz);
   const float bm = below(best);
+  constexpr float kBig = 18446744073709551616.0f;  // 2^64
+  const float ix = __builtin_amdgcn_fmed3f(r.inv.x, -kBig, kBig), iy = __builtin_amdgcn_fmed3f(r.inv.y, -kBig, kBig),
+              iz = __builtin_amdgcn_fmed3f(r.inv.z, -kBig, kBig);
+  const float Ax = sx * ix, Ay = sy * iy, Az = sz * iz;
+  const float Bx = (ox - r.o.x) * ix, By = (oy - r.o.y) * iy, Bz = (oz - r.o.z) * iz;
+  const float ex = fabsf(Bx) * kSlack, ey = fabsf(By) * kSlack, ez = fabsf(Bz) * kSlack;
+  const float Bxl = Bx - ex, Bxh = Bx + ex, Byl = By - ey, Byh = By + ey, Bzl = Bz - ez, Bzh = Bz + ez;
   float t[4];
   bool h[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const float ax = (dequant(ox, nxw, k, sx) - r.o.x) * r.inv.x, bx = (dequant(ox, fxw, k, sx) - r.o.x) * r.inv.x;
-    const float ay = (dequant(oy, nyw, k, sy) - r.o.y) * r.inv.y, by = (dequant(oy, fyw, k, sy) - r.o.y) * r.inv.y;
-    const float az = (dequant(oz, nzw, k, sz) - r.o.z) * r.inv.z, bz = (dequant(oz, fzw, k, sz) - r.o.z) * r.inv.z;
+    const float ax = __builtin_fmaf(plane_q(nxw, k), Ax, Bxl), bx = __builtin_fmaf(plane_q(fxw, k), Ax, Bxh);
+    const float ay = __builtin_fmaf(plane_q(nyw, k), Ay, Byl), by = __builtin_fmaf(plane_q(fyw, k), Ay, Byh);
+    const float az = __builtin_fmaf(plane_q(nzw, k), Az, Bzl), bz = __builtin_fmaf(plane_q(fzw, k), Az, Bzh);
     const float l = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f)) * kWidenLo;
     const float hh = fminf(fminf(bx, by), bz) * kWidenHi;
     t[k] = l;
