@@ -154,7 +154,10 @@ const char *muonx_version(void);
 /* ------------------------------------------------------------ host ingest (CPU)
  * Replaces muon::Parser::Parse (muon/parser.cc:230-880) together with
  * acceleration::BVH::Init (muon/acceleration.cc:204-229): parses a .muon
- * scene and builds the SAH BVH on the host. Needs no GPU. */
+ * scene and builds the SAH BVH on the host. Needs no GPU. `load` reads OBJ
+ * and PLY meshes, relative to the scene file's directory (parse_file) or to
+ * the working directory (parse_text); a file that cannot be read counts as
+ * a warning and is skipped, as the reference logs and skips it. */
 int muonx_host_scene_parse_file(const char *path, const muonx_options *opts, muonx_host_scene **out);
 int muonx_host_scene_parse_text(const char *text, size_t len, const muonx_options *opts,
                                 muonx_host_scene **out);

@@ -89,18 +89,15 @@ int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t) {
   return MUONX_OK;
 }
 
-int muonx_host_scene_parse_text(const char *text, size_t len, const muonx_options *opts, muonx_host_scene **out) {
-  if (!text || !out) {
-    SetError("muonx_host_scene_parse_text: null argument");
-    return MUONX_ERR_INVALID;
-  }
-  *out = nullptr;
+namespace {
+int ParseInto(const std::string &text, const std::string &base_dir, const muonx_options *opts,
+              muonx_host_scene **out) {
   muonx_options o{1, 2, 1, 1, -1};
   if (opts) o = *opts;
   auto *hs = new (std::nothrow) muonx_host_scene();
   if (!hs) return MUONX_ERR_NOMEM;
   try {
-    host::ParseScene(std::string(text, len), o.acceleration, o.partition_strategy, o.default_seed, hs->sc);
+    host::ParseScene(text, o.acceleration, o.partition_strategy, o.default_seed, hs->sc, base_dir);
   } catch (const std::exception &e) {
     delete hs;
     SetError(std::string("parse failed: ") + e.what());
@@ -108,6 +105,16 @@ int muonx_host_scene_parse_text(const char *text, size_t len, const muonx_option
   }
   *out = hs;
   return MUONX_OK;
+}
+}  // namespace
+
+int muonx_host_scene_parse_text(const char *text, size_t len, const muonx_options *opts, muonx_host_scene **out) {
+  if (!text || !out) {
+    SetError("muonx_host_scene_parse_text: null argument");
+    return MUONX_ERR_INVALID;
+  }
+  *out = nullptr;
+  return ParseInto(std::string(text, len), std::string(), opts, out);
 }
 
 int muonx_host_scene_parse_file(const char *path, const muonx_options *opts, muonx_host_scene **out) {
@@ -122,8 +129,12 @@ int muonx_host_scene_parse_file(const char *path, const muonx_options *opts, muo
   }
   std::stringstream ss;
   ss << f.rdbuf();
-  std::string text = ss.str();
-  return muonx_host_scene_parse_text(text.data(), text.size(), opts, out);
+  *out = nullptr;
+  // `load` paths are relative to the scene file (muon/parser.cc:474-477)
+  const std::string p(path);
+  const size_t slash = p.find_last_of('/');
+  const std::string dir = slash == std::string::npos ? std::string(".") : slash == 0 ? std::string("/") : p.substr(0, slash);
+  return ParseInto(ss.str(), dir, opts, out);
 }
 
 void muonx_host_scene_destroy(muonx_host_scene *hs) { delete hs; }

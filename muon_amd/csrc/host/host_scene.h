@@ -57,8 +57,10 @@ struct HostScene {
 
 // Parses .muon text (muon/parser.cc:230-880) and builds the BVH.
 // partition: 0 uniform, 1 midpoint, 2 SAH; acceleration 0 = linear (one leaf).
+// `load` paths are taken relative to base_dir (the scene file's directory;
+// empty: the working directory).
 void ParseScene(const std::string &text, int acceleration, int partition, int64_t default_seed,
-                HostScene &out);
+                HostScene &out, const std::string &base_dir = std::string());
 
 // Builds the binary BVH of muon/acceleration.cc:204-458 into `sc`.
 void BuildBVH(HostScene &sc, int acceleration, int partition);

@@ -1,0 +1,414 @@
+// OBJ / PLY readers for the .muon `load` command; the import rules are listed
+// in mesh_io.h.
+#include "mesh_io.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+
+namespace muonx {
+namespace host {
+namespace {
+
+// Builds one mesh: corners with the same attribute key share a vertex.
+class MeshBuilder {
+ public:
+  explicit MeshBuilder(std::vector<LoadedMesh> &out) : out_(out) {}
+  // key: attribute bits identifying the corner; pos: its position
+  int Corner(const std::vector<uint64_t> &key, const std::array<float, 3> &pos) {
+    auto it = index_.find(key);
+    if (it != index_.end()) return it->second;
+    const int id = int(cur_.pos.size());
+    cur_.pos.push_back(pos);
+    index_.emplace(key, id);
+    return id;
+  }
+  // Adds a polygon as a fan from its first corner, dropping triangles with
+  // two equal corner positions.
+  void Polygon(const std::vector<int> &c) {
+    for (size_t k = 1; k + 1 < c.size(); ++k) {
+      const std::array<int, 3> t{c[0], c[k], c[k + 1]};
+      const auto &a = cur_.pos[size_t(t[0])], &b = cur_.pos[size_t(t[1])], &d = cur_.pos[size_t(t[2])];
+      if (a == b || b == d || a == d) continue;
+      cur_.tris.push_back(t);
+    }
+  }
+  void Finish() {
+    if (!cur_.tris.empty()) out_.push_back(std::move(cur_));
+    cur_ = LoadedMesh();
+    index_.clear();
+  }
+
+ private:
+  std::vector<LoadedMesh> &out_;
+  LoadedMesh cur_;
+  std::map<std::vector<uint64_t>, int> index_;
+};
+
+uint64_t Bits(float f) {
+  if (f == 0.0f) f = 0.0f;  // +0 and -0 are the same value
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof(u));
+  return u;
+}
+
+// ------------------------------------------------------------------ OBJ
+
+// Resolves a 1-based (or negative, counted from the end) OBJ index.
+bool ObjIndex(const std::string &tok, size_t count, long &out) {
+  if (tok.empty()) return false;
+  char *end = nullptr;
+  const long v = std::strtol(tok.c_str(), &end, 10);
+  if (*end != '\0' || v == 0) return false;
+  const long idx = v > 0 ? v - 1 : long(count) + v;
+  if (idx < 0 || size_t(idx) >= count) return false;
+  out = idx;
+  return true;
+}
+
+bool ParseObj(const std::string &data, std::vector<LoadedMesh> &out, std::string &err) {
+  std::vector<std::array<float, 3>> v, vn;
+  std::vector<std::array<float, 3>> vt;
+  MeshBuilder mb(out);
+  std::istringstream in(data);
+  std::string line;
+  size_t lineno = 0;
+  while (std::getline(in, line)) {
+    ++lineno;
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::istringstream ls(line);
+    std::string cmd;
+    if (!(ls >> cmd)) continue;
+    if (cmd == "v" || cmd == "vn") {
+      std::array<float, 3> p{};
+      if (!(ls >> p[0] >> p[1] >> p[2])) {
+        err = "obj line " + std::to_string(lineno) + ": bad " + cmd;
+        return false;
+      }
+      (cmd == "v" ? v : vn).push_back(p);
+    } else if (cmd == "vt") {
+      std::array<float, 3> p{};
+      if (!(ls >> p[0])) {
+        err = "obj line " + std::to_string(lineno) + ": bad vt";
+        return false;
+      }
+      ls >> p[1] >> p[2];  // optional components
+      vt.push_back(p);
+    } else if (cmd == "f") {
+      std::vector<int> corners;
+      std::string tok;
+      while (ls >> tok) {
+        std::string part[3];
+        size_t s0 = tok.find('/');
+        part[0] = tok.substr(0, s0);
+        if (s0 != std::string::npos) {
+          const size_t s1 = tok.find('/', s0 + 1);
+          part[1] = tok.substr(s0 + 1, s1 == std::string::npos ? std::string::npos : s1 - s0 - 1);
+          if (s1 != std::string::npos) part[2] = tok.substr(s1 + 1);
+        }
+        long iv, it = -1, in_ = -1;
+        if (!ObjIndex(part[0], v.size(), iv) || (!part[1].empty() && !ObjIndex(part[1], vt.size(), it)) ||
+            (!part[2].empty() && !ObjIndex(part[2], vn.size(), in_))) {
+          err = "obj line " + std::to_string(lineno) + ": bad face index '" + tok + "'";
+          return false;
+        }
+        std::vector<uint64_t> key;
+        for (float c : v[size_t(iv)]) key.push_back(Bits(c));
+        key.push_back(it >= 0 ? 1u : 0u);
+        if (it >= 0)
+          for (float c : vt[size_t(it)]) key.push_back(Bits(c));
+        key.push_back(in_ >= 0 ? 1u : 0u);
+        if (in_ >= 0)
+          for (float c : vn[size_t(in_)]) key.push_back(Bits(c));
+        corners.push_back(mb.Corner(key, v[size_t(iv)]));
+      }
+      if (corners.size() < 3) {
+        err = "obj line " + std::to_string(lineno) + ": face with fewer than 3 corners";
+        return false;
+      }
+      mb.Polygon(corners);
+    } else if (cmd == "o" || cmd == "g" || cmd == "usemtl") {
+      mb.Finish();
+    }
+    // other statements (mtllib, s, l, p, ...) carry nothing muon uses
+  }
+  mb.Finish();
+  return true;
+}
+
+// ------------------------------------------------------------------ PLY
+
+enum class PlyType { kI8, kU8, kI16, kU16, kI32, kU32, kF32, kF64, kBad };
+
+PlyType PlyTypeOf(const std::string &s) {
+  if (s == "char" || s == "int8") return PlyType::kI8;
+  if (s == "uchar" || s == "uint8") return PlyType::kU8;
+  if (s == "short" || s == "int16") return PlyType::kI16;
+  if (s == "ushort" || s == "uint16") return PlyType::kU16;
+  if (s == "int" || s == "int32") return PlyType::kI32;
+  if (s == "uint" || s == "uint32") return PlyType::kU32;
+  if (s == "float" || s == "float32") return PlyType::kF32;
+  if (s == "double" || s == "float64") return PlyType::kF64;
+  return PlyType::kBad;
+}
+
+size_t PlySize(PlyType t) {
+  switch (t) {
+    case PlyType::kI8: case PlyType::kU8: return 1;
+    case PlyType::kI16: case PlyType::kU16: return 2;
+    case PlyType::kI32: case PlyType::kU32: case PlyType::kF32: return 4;
+    case PlyType::kF64: return 8;
+    default: return 0;
+  }
+}
+
+struct PlyProp {
+  std::string name;
+  bool list = false;
+  PlyType count_type = PlyType::kBad, type = PlyType::kBad;
+};
+
+struct PlyElement {
+  std::string name;
+  size_t count = 0;
+  std::vector<PlyProp> props;
+};
+
+// Reads values of a PLY body: ASCII tokens or binary in either byte order.
+class PlyReader {
+ public:
+  PlyReader(const std::string &data, size_t pos, int format) : d_(data), p_(pos), fmt_(format) {}
+  bool Read(PlyType t, double &out) {
+    if (fmt_ == 0) {
+      while (p_ < d_.size() && std::isspace(static_cast<unsigned char>(d_[p_]))) ++p_;
+      const size_t b = p_;
+      while (p_ < d_.size() && !std::isspace(static_cast<unsigned char>(d_[p_]))) ++p_;
+      if (b == p_) return false;
+      const std::string tok = d_.substr(b, p_ - b);
+      char *end = nullptr;
+      out = std::strtod(tok.c_str(), &end);
+      return *end == '\0';
+    }
+    const size_t n = PlySize(t);
+    if (n == 0 || p_ + n > d_.size()) return false;
+    unsigned char b[8];
+    std::memcpy(b, d_.data() + p_, n);
+    p_ += n;
+    if (fmt_ == 2) std::reverse(b, b + n);  // big endian -> host (little endian)
+    switch (t) {
+      case PlyType::kI8: { int8_t v; std::memcpy(&v, b, 1); out = v; break; }
+      case PlyType::kU8: { uint8_t v; std::memcpy(&v, b, 1); out = v; break; }
+      case PlyType::kI16: { int16_t v; std::memcpy(&v, b, 2); out = v; break; }
+      case PlyType::kU16: { uint16_t v; std::memcpy(&v, b, 2); out = v; break; }
+      case PlyType::kI32: { int32_t v; std::memcpy(&v, b, 4); out = v; break; }
+      case PlyType::kU32: { uint32_t v; std::memcpy(&v, b, 4); out = v; break; }
+      case PlyType::kF32: { float v; std::memcpy(&v, b, 4); out = v; break; }
+      case PlyType::kF64: { double v; std::memcpy(&v, b, 8); out = v; break; }
+      default: return false;
+    }
+    return true;
+  }
+
+ private:
+  const std::string &d_;
+  size_t p_;
+  int fmt_;  // 0 ascii, 1 little endian, 2 big endian
+};
+
+bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string &err) {
+  std::istringstream hs(data);
+  std::string line;
+  if (!std::getline(hs, line) || line.substr(0, 3) != "ply") {
+    err = "ply: missing magic";
+    return false;
+  }
+  int fmt = -1;
+  std::vector<PlyElement> elems;
+  bool ended = false;
+  while (std::getline(hs, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    std::istringstream ls(line);
+    std::string kw;
+    if (!(ls >> kw)) continue;
+    if (kw == "format") {
+      std::string f;
+      ls >> f;
+      fmt = f == "ascii" ? 0 : f == "binary_little_endian" ? 1 : f == "binary_big_endian" ? 2 : -1;
+    } else if (kw == "element") {
+      PlyElement e;
+      if (!(ls >> e.name >> e.count)) {
+        err = "ply: bad element line";
+        return false;
+      }
+      elems.push_back(e);
+    } else if (kw == "property") {
+      if (elems.empty()) {
+        err = "ply: property before element";
+        return false;
+      }
+      PlyProp p;
+      std::string t;
+      ls >> t;
+      if (t == "list") {
+        std::string ct, it;
+        ls >> ct >> it >> p.name;
+        p.list = true;
+        p.count_type = PlyTypeOf(ct);
+        p.type = PlyTypeOf(it);
+        if (p.count_type == PlyType::kBad || p.type == PlyType::kBad) {
+          err = "ply: bad list property types";
+          return false;
+        }
+      } else {
+        p.type = PlyTypeOf(t);
+        ls >> p.name;
+        if (p.type == PlyType::kBad) {
+          err = "ply: bad property type '" + t + "'";
+          return false;
+        }
+      }
+      elems.back().props.push_back(p);
+    } else if (kw == "end_header") {
+      ended = true;
+      break;
+    }
+    // comment / obj_info lines are skipped
+  }
+  if (!ended || fmt < 0) {
+    err = "ply: incomplete header or unknown format";
+    return false;
+  }
+  const std::streamoff body = hs.tellg();
+  if (body < 0) {
+    err = "ply: no body";
+    return false;
+  }
+  PlyReader rd(data, size_t(body), fmt);
+  std::vector<std::vector<uint64_t>> vkey;
+  std::vector<std::array<float, 3>> vpos;
+  std::vector<std::vector<long>> faces;
+  for (const PlyElement &e : elems) {
+    const bool is_vertex = e.name == "vertex", is_face = e.name == "face";
+    int ix = -1, iy = -1, iz = -1, ifl = -1;
+    for (size_t k = 0; k < e.props.size(); ++k) {
+      const PlyProp &p = e.props[k];
+      if (p.list) {
+        if (p.name == "vertex_indices" || p.name == "vertex_index") ifl = int(k);
+      } else if (p.name == "x") ix = int(k);
+      else if (p.name == "y") iy = int(k);
+      else if (p.name == "z") iz = int(k);
+    }
+    if (is_vertex && (ix < 0 || iy < 0 || iz < 0)) {
+      err = "ply: vertex element without x, y, z";
+      return false;
+    }
+    if (is_face && ifl < 0) {
+      err = "ply: face element without vertex_indices";
+      return false;
+    }
+    for (size_t r = 0; r < e.count; ++r) {
+      std::vector<uint64_t> key;
+      std::array<float, 3> pos{};
+      std::vector<long> face;
+      for (size_t k = 0; k < e.props.size(); ++k) {
+        const PlyProp &p = e.props[k];
+        double v;
+        if (p.list) {
+          double cnt;
+          if (!rd.Read(p.count_type, cnt) || cnt < 0) {
+            err = "ply: truncated list";
+            return false;
+          }
+          for (long j = 0; j < long(cnt); ++j) {
+            if (!rd.Read(p.type, v)) {
+              err = "ply: truncated list";
+              return false;
+            }
+            if (is_face && int(k) == ifl) face.push_back(long(v));
+          }
+          continue;
+        }
+        if (!rd.Read(p.type, v)) {
+          err = "ply: truncated element '" + e.name + "'";
+          return false;
+        }
+        if (is_vertex) {
+          const float f = float(v);
+          key.push_back(Bits(f));
+          if (int(k) == ix) pos[0] = f;
+          else if (int(k) == iy) pos[1] = f;
+          else if (int(k) == iz) pos[2] = f;
+        }
+      }
+      if (is_vertex) {
+        vkey.push_back(std::move(key));
+        vpos.push_back(pos);
+      } else if (is_face) {
+        faces.push_back(std::move(face));
+      }
+    }
+  }
+  MeshBuilder mb(out);
+  std::vector<int> joined(vpos.size());
+  for (size_t i = 0; i < vpos.size(); ++i) joined[i] = mb.Corner(vkey[i], vpos[i]);
+  for (const auto &f : faces) {
+    if (f.size() < 3) {
+      err = "ply: face with fewer than 3 corners";
+      return false;
+    }
+    std::vector<int> c;
+    for (long i : f) {
+      if (i < 0 || size_t(i) >= vpos.size()) {
+        err = "ply: face index out of range";
+        return false;
+      }
+      c.push_back(joined[size_t(i)]);
+    }
+    mb.Polygon(c);
+  }
+  mb.Finish();
+  return true;
+}
+
+std::string Lower(std::string s) {
+  for (char &c : s) c = char(std::tolower(static_cast<unsigned char>(c)));
+  return s;
+}
+
+}  // namespace
+
+bool LoadMeshData(const std::string &data, const std::string &ext, std::vector<LoadedMesh> &out,
+                  std::string &err) {
+  std::vector<LoadedMesh> meshes;
+  const std::string e = Lower(ext);
+  bool ok;
+  if (e == "obj") ok = ParseObj(data, meshes, err);
+  else if (e == "ply") ok = ParsePly(data, meshes, err);
+  else {
+    err = "unsupported mesh format '" + ext + "' (obj and ply are read)";
+    return false;
+  }
+  if (ok) out = std::move(meshes);
+  return ok;
+}
+
+bool LoadMeshFile(const std::string &path, std::vector<LoadedMesh> &out, std::string &err) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) {
+    err = "cannot open mesh file: " + path;
+    return false;
+  }
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const size_t dot = path.find_last_of('.');
+  const std::string ext = dot == std::string::npos ? std::string() : path.substr(dot + 1);
+  return LoadMeshData(ss.str(), ext, out, err);
+}
+
+}  // namespace host
+}  // namespace muonx

@@ -14,6 +14,7 @@
 #include <unordered_map>
 
 #include "host_scene.h"
+#include "mesh_io.h"
 #include "xform.h"
 
 namespace muonx {
@@ -87,7 +88,7 @@ bool ParseOnOff(std::istringstream &iss, bool &out) {
 }  // namespace
 
 void ParseScene(const std::string &text, int acceleration, int partition, int64_t default_seed,
-                HostScene &sc) {
+                HostScene &sc, const std::string &base_dir) {
   std::mt19937 seedgen(default_seed >= 0 ? static_cast<uint32_t>(default_seed) : std::random_device()());
   MaterialState mat;
   std::vector<M4> stack{Identity()};
@@ -107,6 +108,9 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
   };
   std::vector<std::deque<Vertex>> meshes(1);
   int cur_mesh = 0;
+  // vertices of loaded meshes: outside `meshes`, so the final normalisation
+  // pass does not reach them (muon/scene.cc:9-14, muon/parser.cc:862-871)
+  std::deque<Vertex> gen;
   std::vector<TriRef> tri_refs;
   float attenuation[3] = {1.0f, 0.0f, 0.0f};
 
@@ -218,7 +222,37 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
       stack.push_back(stack.back());
       update_cached();
     } else if (cmd == "load") {
-      bad();  // assimp import is not part of this render path
+      // muon/parser.cc:465-561: every triangle of the file's meshes, with the
+      // current transform and material; a file that cannot be read is
+      // logged and skipped
+      std::string fname;
+      iss >> fname;
+      if (iss.fail()) { bad(); continue; }
+      const std::string path = (base_dir.empty() || fname[0] == '/') ? fname : base_dir + "/" + fname;
+      std::vector<LoadedMesh> loaded;
+      std::string err;
+      if (!LoadMeshFile(path, loaded, err)) { bad(); continue; }
+      for (const LoadedMesh &lm : loaded) {
+        const size_t base = gen.size();
+        for (const auto &p : lm.pos) gen.push_back(Vertex{V3{p[0], p[1], p[2]}, V3{0, 0, 0}});
+        for (const auto &t : lm.tris) {
+          const size_t ia = base + size_t(t[0]), ib = base + size_t(t[1]), ic = base + size_t(t[2]);
+          Vertex *vs[3] = {&gen[ia], &gen[ib], &gen[ic]};
+          float g[9];
+          for (int k = 0; k < 3; ++k) {
+            g[3 * k] = vs[k]->pos.x;
+            g[3 * k + 1] = vs[k]->pos.y;
+            g[3 * k + 2] = vs[k]->pos.z;
+          }
+          int id = add_prim(MUONX_PRIM_TRI, g, 9, current_xform(), mats.Get(mat.Record()), -1);
+          if (sc.compute_vertex_normals) {
+            const V3 n = cross(vs[1]->pos - vs[0]->pos, vs[2]->pos - vs[0]->pos);
+            for (Vertex *v : vs) v->normal = v->normal + n;
+            sc.prim_use_vnormal[id] = 1;
+          }
+          tri_refs.push_back(TriRef{id, -1, {int(ia), int(ib), int(ic)}});
+        }
+      }
     } else if (cmd == "sphere") {
       float g[4];
       iss >> g[0] >> g[1] >> g[2] >> g[3];
@@ -352,7 +386,7 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
   }
   for (const TriRef &t : tri_refs) {
     for (int k = 0; k < 3; ++k) {
-      const Vertex &v = meshes[t.mesh][t.idx[k]];
+      const Vertex &v = t.mesh < 0 ? gen[size_t(t.idx[k])] : meshes[t.mesh][t.idx[k]];
       sc.prim_vnormal[t.prim * 9 + 3 * k] = v.normal.x;
       sc.prim_vnormal[t.prim * 9 + 3 * k + 1] = v.normal.y;
       sc.prim_vnormal[t.prim * 9 + 3 * k + 2] = v.normal.z;
