@@ -5,8 +5,10 @@
 // called with the same predicates on the same input order, so the tree (and
 // the leaf order of the primitives) is the reference's.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 #include "host_scene.h"
 #include "xform.h"
@@ -88,15 +90,57 @@ Box WorldBox(const HostScene &sc, size_t i) {
   return b.Add(xform_point(m, hi));
 }
 
+// Nodes of a subtree in pre-order; child links are indices into the same
+// buffer until the buffer is spliced into its parent's.
+struct NodeBuf {
+  std::vector<float> bounds;
+  std::vector<int32_t> left, right, axis, start, count;
+  int depth = 0;
+  int New() {
+    const int id = int(left.size());
+    left.push_back(-1);
+    right.push_back(-1);
+    axis.push_back(0);
+    start.push_back(0);
+    count.push_back(0);
+    bounds.resize(bounds.size() + 6, 0.0f);
+    return id;
+  }
+  void SetBounds(int id, const Box &b) {
+    float *p = &bounds[size_t(id) * 6];
+    p[0] = b.mn.x; p[1] = b.mn.y; p[2] = b.mn.z;
+    p[3] = b.mx.x; p[4] = b.mx.y; p[5] = b.mx.z;
+  }
+  // Appends `o` (a complete subtree); returns the index of its root here.
+  int Append(const NodeBuf &o) {
+    const int off = int(left.size());
+    for (size_t i = 0; i < o.left.size(); ++i) {
+      left.push_back(o.left[i] >= 0 ? o.left[i] + off : -1);
+      right.push_back(o.right[i] >= 0 ? o.right[i] + off : -1);
+    }
+    axis.insert(axis.end(), o.axis.begin(), o.axis.end());
+    start.insert(start.end(), o.start.begin(), o.start.end());
+    count.insert(count.end(), o.count.begin(), o.count.end());
+    bounds.insert(bounds.end(), o.bounds.begin(), o.bounds.end());
+    depth = std::max(depth, o.depth);
+    return off;
+  }
+};
+
 class Builder {
  public:
-  Builder(HostScene &sc, int partition, std::vector<PrimInfo> &info) : sc_(sc), partition_(partition), info_(info) {}
+  Builder(int partition, std::vector<PrimInfo> &info, int threads)
+      : partition_(partition), info_(info), spare_(threads - 1) {}
 
-  // Emits the subtree over info_[start, end) in pre-order; returns its index.
-  int Build(size_t start, size_t end, int depth, Box *out_box) {
-    sc_.bvh_depth = std::max(sc_.bvh_depth, depth);
+  // Emits the subtree over info_[start, end) in pre-order into `buf`;
+  // returns its root's index there. When both halves of a split are large
+  // and a thread is spare, the right half is built on its own thread into
+  // its own buffer; both halves are appended in pre-order, so the node
+  // numbering is the sequential build's.
+  int Build(NodeBuf &buf, size_t start, size_t end, int depth, Box *out_box) {
+    buf.depth = std::max(buf.depth, depth);
     const size_t n = end - start;
-    if (n == 1) return Leaf(start, n, info_[start].box, out_box);
+    if (n == 1) return Leaf(buf, start, n, info_[start].box, out_box);
 
     Box cb;
     for (size_t i = start; i < end; ++i) cb = cb.Add(info_[i].centroid);
@@ -158,7 +202,7 @@ class Builder {
         }
       }
       const float leaf_cost = float(n);
-      if (leaf_cost < min_cost) return Leaf(start, n, all, out_box);
+      if (leaf_cost < min_cost) return Leaf(buf, start, n, all, out_box);
       auto it = std::partition(first, last, [&](const PrimInfo &p) { return bucket_of(p) <= best; });
       split = size_t(it - info_.begin());
       uniform = (split == start || split == end);
@@ -169,47 +213,54 @@ class Builder {
         return comp(a.centroid, axis) < comp(b.centroid, axis);
       });
     }
-    const int me = NewNode();
+    const int me = buf.New();
     Box lb, rb;
-    const int l = Build(start, split, depth + 1, &lb);
-    const int r = Build(split, end, depth + 1, &rb);
+    int l, r;
+    if (std::min(split - start, end - split) >= kSpawnPrims && TakeThread()) {
+      // the two halves touch disjoint ranges of info_
+      NodeBuf lbuf, rbuf;
+      int lroot = 0, rroot = 0;
+      std::thread t([&] { rroot = Build(rbuf, split, end, depth + 1, &rb); });
+      lroot = Build(lbuf, start, split, depth + 1, &lb);
+      t.join();
+      spare_.fetch_add(1);
+      l = buf.Append(lbuf) + lroot;
+      r = buf.Append(rbuf) + rroot;
+    } else {
+      l = Build(buf, start, split, depth + 1, &lb);
+      r = Build(buf, split, end, depth + 1, &rb);
+    }
     Box b = lb.Add(rb);
-    sc_.node_left[me] = l;
-    sc_.node_right[me] = r;
-    sc_.node_axis[me] = axis;
-    SetBounds(me, b);
+    buf.left[size_t(me)] = l;
+    buf.right[size_t(me)] = r;
+    buf.axis[size_t(me)] = axis;
+    buf.SetBounds(me, b);
     *out_box = b;
     return me;
   }
 
  private:
-  int NewNode() {
-    int id = int(sc_.node_left.size());
-    sc_.node_left.push_back(-1);
-    sc_.node_right.push_back(-1);
-    sc_.node_axis.push_back(0);
-    sc_.node_start.push_back(0);
-    sc_.node_count.push_back(0);
-    for (int k = 0; k < 6; ++k) sc_.node_bounds.push_back(0.0f);
-    return id;
+  static constexpr size_t kSpawnPrims = 1 << 15;
+
+  bool TakeThread() {
+    int s = spare_.load();
+    while (s > 0)
+      if (spare_.compare_exchange_weak(s, s - 1)) return true;
+    return false;
   }
-  void SetBounds(int id, const Box &b) {
-    float *p = &sc_.node_bounds[size_t(id) * 6];
-    p[0] = b.mn.x; p[1] = b.mn.y; p[2] = b.mn.z;
-    p[3] = b.mx.x; p[4] = b.mx.y; p[5] = b.mx.z;
-  }
-  int Leaf(size_t start, size_t n, const Box &b, Box *out_box) {
-    int id = NewNode();
-    sc_.node_start[id] = int32_t(start);
-    sc_.node_count[id] = int32_t(n);
-    SetBounds(id, b);
+
+  int Leaf(NodeBuf &buf, size_t start, size_t n, const Box &b, Box *out_box) {
+    int id = buf.New();
+    buf.start[size_t(id)] = int32_t(start);
+    buf.count[size_t(id)] = int32_t(n);
+    buf.SetBounds(id, b);
     *out_box = b;
     return id;
   }
 
-  HostScene &sc_;
   int partition_;
   std::vector<PrimInfo> &info_;
+  std::atomic<int> spare_;
 };
 
 }  // namespace
@@ -225,12 +276,21 @@ void BuildBVH(HostScene &sc, int acceleration, int partition) {
   sc.leaf_prims.clear();
   sc.bvh_depth = 0;
   if (n == 0) return;
-  std::vector<PrimInfo> info;
-  info.reserve(n);
-  for (size_t i = 0; i < n; ++i) {
-    Box b = WorldBox(sc, i);
-    V3 diag = b.mx - b.mn;
-    info.push_back(PrimInfo{i, b, b.mn + 0.5f * diag});
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<PrimInfo> info(n);
+  {
+    auto boxes = [&](size_t lo, size_t hi) {
+      for (size_t i = lo; i < hi; ++i) {
+        Box b = WorldBox(sc, i);
+        V3 diag = b.mx - b.mn;
+        info[i] = PrimInfo{i, b, b.mn + 0.5f * diag};
+      }
+    };
+    const size_t nt = n >= (size_t(1) << 16) ? hw : 1;
+    std::vector<std::thread> ts;
+    for (size_t t = 1; t < nt; ++t) ts.emplace_back(boxes, n * t / nt, n * (t + 1) / nt);
+    boxes(0, n / nt);
+    for (auto &t : ts) t.join();
   }
   if (acceleration == 0) {
     // Linear structure (muon/acceleration.cc:14-45): every primitive in parse
@@ -246,9 +306,17 @@ void BuildBVH(HostScene &sc, int acceleration, int partition) {
     for (size_t i = 0; i < n; ++i) sc.leaf_prims.push_back(int32_t(i));
     return;
   }
-  Builder b(sc, partition, info);
+  Builder b(partition, info, int(hw));
+  NodeBuf nodes;
   Box root;
-  b.Build(0, n, 1, &root);
+  b.Build(nodes, 0, n, 1, &root);
+  sc.node_bounds = std::move(nodes.bounds);
+  sc.node_left = std::move(nodes.left);
+  sc.node_right = std::move(nodes.right);
+  sc.node_axis = std::move(nodes.axis);
+  sc.node_start = std::move(nodes.start);
+  sc.node_count = std::move(nodes.count);
+  sc.bvh_depth = nodes.depth;
   sc.leaf_prims.resize(n);
   for (size_t i = 0; i < n; ++i) sc.leaf_prims[i] = int32_t(info[i].index);
 }

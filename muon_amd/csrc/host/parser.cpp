@@ -6,7 +6,12 @@
 // (:165-178), quad_light adds two identity-transform emissive tris (:736-763).
 #include <algorithm>
 #include <cctype>
+#include <cerrno>
+#include <climits>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <deque>
 #include <map>
 #include <random>
@@ -68,11 +73,119 @@ struct MaterialState {
   }
 };
 
-void TrimInPlace(std::string &s) {
-  size_t b = 0, e = s.size();
-  while (b < e && std::isspace(static_cast<unsigned char>(s[b]))) ++b;
-  while (e > b && std::isspace(static_cast<unsigned char>(s[e - 1]))) --e;
-  s = s.substr(b, e - b);
+// Decimal token -> float with the correctly rounded result of strtof, for
+// tokens [sign] digits [. digits] [(e|E) [sign] digits] whose significant
+// digits fit 2^53 and whose decimal exponent is within +-22: the digits are
+// an exact double, 10^|k| is an exact double, so one multiplication or
+// division gives the correctly rounded double d of the value; rounding d to
+// float gives the correctly rounded float unless d lies exactly halfway
+// between two floats (a float midpoint is a double, so the value and d
+// cannot straddle one otherwise). Returns false for every other token and
+// for that halfway case.
+bool FastDecimal(const char *b, size_t len, float &out) {
+  static const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                                    1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+  size_t i = 0;
+  bool neg = false;
+  if (i < len && (b[i] == '+' || b[i] == '-')) neg = b[i++] == '-';
+  uint64_t m = 0;
+  int digits = 0, frac = 0;
+  bool any = false;
+  while (i < len && b[i] >= '0' && b[i] <= '9') {
+    any = true;
+    if (m || b[i] != '0') {
+      if (++digits > 17) return false;
+      m = m * 10 + uint64_t(b[i] - '0');
+    }
+    ++i;
+  }
+  if (i < len && b[i] == '.') {
+    ++i;
+    while (i < len && b[i] >= '0' && b[i] <= '9') {
+      any = true;
+      if (m || b[i] != '0') {
+        if (++digits > 17) return false;
+        m = m * 10 + uint64_t(b[i] - '0');
+      }
+      ++frac;
+      ++i;
+    }
+  }
+  if (!any) return false;
+  int ex = 0;
+  if (i < len && (b[i] == 'e' || b[i] == 'E')) {
+    ++i;
+    bool eneg = false;
+    if (i < len && (b[i] == '+' || b[i] == '-')) eneg = b[i++] == '-';
+    if (i == len) return false;
+    while (i < len && b[i] >= '0' && b[i] <= '9') {
+      if (ex > 1000) return false;
+      ex = ex * 10 + (b[i] - '0');
+      ++i;
+    }
+    if (eneg) ex = -ex;
+  }
+  if (i != len || m >= (uint64_t(1) << 53)) return false;
+  const int k = ex - frac;  // value = m * 10^k
+  if (k < -22 || k > 22) return false;
+  double d = double(m);
+  d = k >= 0 ? d * kPow10[k] : d / kPow10[-k];
+  uint64_t bits;
+  std::memcpy(&bits, &d, sizeof(bits));
+  if (m != 0 && (bits & ((uint64_t(1) << 29) - 1)) == (uint64_t(1) << 28)) return false;  // float midpoint
+  const float f = float(d);
+  out = neg ? -f : f;
+  return true;
+}
+
+// Reads `n` whitespace-separated numbers from [s, e) for the hot commands
+// (vertex, tri) without building a stream. A token is accepted only when it
+// is made of digits, signs, '.', 'e' / 'E', and strtof / strtol reads all of
+// it without a range error; it then has the value std::istream reads. On
+// anything else the caller takes the istream path, which decides as before.
+// Content after the n-th number is ignored, as the istream path ignores it.
+template <class T>
+bool FastNumbers(const char *s, const char *e, T *out, int n) {
+  for (int k = 0; k < n; ++k) {
+    while (s < e && std::isspace(static_cast<unsigned char>(*s))) ++s;
+    const char *b = s;
+    while (s < e && !std::isspace(static_cast<unsigned char>(*s))) {
+      const char c = *s;
+      const bool ok = (c >= '0' && c <= '9') || c == '+' || c == '-' ||
+                      (std::is_floating_point<T>::value && (c == '.' || c == 'e' || c == 'E'));
+      if (!ok) return false;
+      ++s;
+    }
+    const size_t len = size_t(s - b);
+    if (len == 0 || len > 63) return false;
+    if constexpr (std::is_floating_point<T>::value) {
+      if (FastDecimal(b, len, out[k])) continue;
+    } else if (len <= 9 && b[0] != '+' && b[0] != '-') {
+      // up to 9 plain digits: below 2^31, read exactly
+      int64_t v = 0;
+      size_t j = 0;
+      while (j < len && b[j] >= '0' && b[j] <= '9') v = v * 10 + (b[j++] - '0');
+      if (j == len) {
+        out[k] = T(v);
+        continue;
+      }
+    }
+    char buf[64];
+    std::memcpy(buf, b, len);
+    buf[len] = '\0';
+    char *end = nullptr;
+    errno = 0;
+    if constexpr (std::is_floating_point<T>::value) {
+      const float v = std::strtof(buf, &end);
+      if (end != buf + len || errno == ERANGE) return false;
+      out[k] = v;
+    } else {
+      const long v = std::strtol(buf, &end, 10);
+      if (end != buf + len || errno == ERANGE || v < INT_MIN || v > INT_MAX) return false;
+      out[k] = T(v);
+    }
+  }
+  return true;
 }
 
 bool ParseOnOff(std::istringstream &iss, bool &out) {
@@ -93,18 +206,29 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
   MaterialState mat;
   std::vector<M4> stack{Identity()};
   M4 inv = Identity(), inv_t = Identity();
+  // record ids of the current transform and material, looked up again only
+  // after the state changes
+  int32_t xf_id = -1, mat_id = -1;
   auto update_cached = [&]() {
     inv = Inverse(stack.back());
     inv_t = Transpose(inv);
+    xf_id = -1;
   };
   RecordTable<48> xforms{&sc.transforms, {}};
   RecordTable<16> mats{&sc.materials, {}};
   auto current_xform = [&]() {
-    std::array<float, 48> r;
-    std::copy(stack.back().begin(), stack.back().end(), r.begin());
-    std::copy(inv.begin(), inv.end(), r.begin() + 16);
-    std::copy(inv_t.begin(), inv_t.end(), r.begin() + 32);
-    return xforms.Get(r);
+    if (xf_id < 0) {
+      std::array<float, 48> r;
+      std::copy(stack.back().begin(), stack.back().end(), r.begin());
+      std::copy(inv.begin(), inv.end(), r.begin() + 16);
+      std::copy(inv_t.begin(), inv_t.end(), r.begin() + 32);
+      xf_id = xforms.Get(r);
+    }
+    return xf_id;
+  };
+  auto current_mat = [&]() {
+    if (mat_id < 0) mat_id = mats.Get(mat.Record());
+    return mat_id;
   };
   std::vector<std::deque<Vertex>> meshes(1);
   int cur_mesh = 0;
@@ -114,11 +238,32 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
   std::vector<TriRef> tri_refs;
   float attenuation[3] = {1.0f, 0.0f, 0.0f};
 
+  {
+    // capacity for the primitives of tri / sphere lines (a hint: quad
+    // lights and loaded meshes add more)
+    size_t est = 0;
+    for (size_t pos = 0; pos < text.size();) {
+      size_t q = pos;
+      while (q < text.size() && (text[q] == ' ' || text[q] == '\t')) ++q;
+      if (text.compare(q, 4, "tri ") == 0 || text.compare(q, 7, "sphere ") == 0) ++est;
+      const size_t nl = text.find('\n', pos);
+      pos = nl == std::string::npos ? text.size() : nl + 1;
+    }
+    sc.prim_type.reserve(est);
+    sc.prim_geom.reserve(12 * est);
+    sc.prim_vnormal.reserve(9 * est);
+    sc.prim_use_vnormal.reserve(est);
+    sc.prim_transform.reserve(est);
+    sc.prim_material.reserve(est);
+    sc.prim_light.reserve(est);
+  }
   auto add_prim = [&](uint8_t type, const float *geom, int ngeom, int32_t xf, int32_t m, int32_t light) {
     int id = int(sc.prim_type.size());
     sc.prim_type.push_back(type);
-    for (int i = 0; i < 12; ++i) sc.prim_geom.push_back(i < ngeom ? geom[i] : 0.0f);
-    for (int i = 0; i < 9; ++i) sc.prim_vnormal.push_back(0.0f);
+    const size_t g0 = sc.prim_geom.size();
+    sc.prim_geom.resize(g0 + 12, 0.0f);
+    std::memcpy(sc.prim_geom.data() + g0, geom, sizeof(float) * size_t(std::min(ngeom, 12)));
+    sc.prim_vnormal.resize(sc.prim_vnormal.size() + 9, 0.0f);
     sc.prim_use_vnormal.push_back(0);
     sc.prim_transform.push_back(xf);
     sc.prim_material.push_back(m);
@@ -126,12 +271,63 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
     return id;
   };
   auto bad = [&]() { ++sc.warnings; };
+  // tri a b c on the current mesh; false for an index out of range
+  auto add_tri = [&](int a, int b, int c) {
+    auto &verts = meshes[cur_mesh];
+    const int nv = int(verts.size());
+    if (a < 0 || b < 0 || c < 0 || a >= nv || b >= nv || c >= nv) return false;
+    float g[9];
+    const Vertex *vs[3] = {&verts[a], &verts[b], &verts[c]};
+    for (int k = 0; k < 3; ++k) {
+      g[3 * k] = vs[k]->pos.x;
+      g[3 * k + 1] = vs[k]->pos.y;
+      g[3 * k + 2] = vs[k]->pos.z;
+    }
+    int id = add_prim(MUONX_PRIM_TRI, g, 9, current_xform(), current_mat(), -1);
+    if (sc.compute_vertex_normals) {
+      // muon/parser.cc:630-637: accumulate the unnormalised face normal.
+      V3 n = cross(vs[1]->pos - vs[0]->pos, vs[2]->pos - vs[0]->pos);
+      verts[a].normal = verts[a].normal + n;
+      verts[b].normal = verts[b].normal + n;
+      verts[c].normal = verts[c].normal + n;
+      sc.prim_use_vnormal[id] = 1;
+    }
+    tri_refs.push_back(TriRef{id, cur_mesh, {a, b, c}});
+    return true;
+  };
 
-  std::istringstream in(text);
+  // lines as std::getline splits them, trimmed; vertex and tri lines of
+  // plain numbers skip the stream (large meshes are millions of them)
+  const size_t tn = text.size();
+  size_t next = 0;
   std::string line;
-  while (std::getline(in, line)) {
-    TrimInPlace(line);
-    if (line.empty() || line[0] == '#') continue;
+  while (next < tn) {
+    size_t lb = next, le = text.find('\n', next);
+    if (le == std::string::npos) le = tn;
+    next = le + 1;
+    while (lb < le && std::isspace(static_cast<unsigned char>(text[lb]))) ++lb;
+    while (le > lb && std::isspace(static_cast<unsigned char>(text[le - 1]))) --le;
+    if (lb == le || text[lb] == '#') continue;
+    {
+      const char *p = text.data() + lb, *pe = text.data() + le;
+      const char *ce = p;
+      while (ce < pe && !std::isspace(static_cast<unsigned char>(*ce))) ++ce;
+      const size_t cl = size_t(ce - p);
+      if (cl == 6 && std::memcmp(p, "vertex", 6) == 0) {
+        float x[3];
+        if (FastNumbers(ce, pe, x, 3)) {
+          meshes[cur_mesh].push_back(Vertex{V3{x[0], x[1], x[2]}, V3{0, 0, 0}});
+          continue;
+        }
+      } else if (cl == 3 && std::memcmp(p, "tri", 3) == 0) {
+        int t[3];
+        if (FastNumbers(ce, pe, t, 3)) {
+          if (!add_tri(t[0], t[1], t[2])) bad();
+          continue;
+        }
+      }
+    }
+    line.assign(text, lb, le - lb);
     std::istringstream iss(line);
     std::string cmd;
     iss >> cmd;
@@ -244,7 +440,7 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
             g[3 * k + 1] = vs[k]->pos.y;
             g[3 * k + 2] = vs[k]->pos.z;
           }
-          int id = add_prim(MUONX_PRIM_TRI, g, 9, current_xform(), mats.Get(mat.Record()), -1);
+          int id = add_prim(MUONX_PRIM_TRI, g, 9, current_xform(), current_mat(), -1);
           if (sc.compute_vertex_normals) {
             const V3 n = cross(vs[1]->pos - vs[0]->pos, vs[2]->pos - vs[0]->pos);
             for (Vertex *v : vs) v->normal = v->normal + n;
@@ -257,7 +453,7 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
       float g[4];
       iss >> g[0] >> g[1] >> g[2] >> g[3];
       if (iss.fail()) { bad(); continue; }
-      add_prim(MUONX_PRIM_SPHERE, g, 4, current_xform(), mats.Get(mat.Record()), -1);
+      add_prim(MUONX_PRIM_SPHERE, g, 4, current_xform(), current_mat(), -1);
     } else if (cmd == "start_mesh") {
       meshes.emplace_back();
       cur_mesh = int(meshes.size()) - 1;
@@ -273,27 +469,7 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
     } else if (cmd == "tri") {
       int a, b, c;
       iss >> a >> b >> c;
-      if (iss.fail()) { bad(); continue; }
-      auto &verts = meshes[cur_mesh];
-      const int nv = int(verts.size());
-      if (a < 0 || b < 0 || c < 0 || a >= nv || b >= nv || c >= nv) { bad(); continue; }
-      float g[9];
-      const Vertex *vs[3] = {&verts[a], &verts[b], &verts[c]};
-      for (int k = 0; k < 3; ++k) {
-        g[3 * k] = vs[k]->pos.x;
-        g[3 * k + 1] = vs[k]->pos.y;
-        g[3 * k + 2] = vs[k]->pos.z;
-      }
-      int id = add_prim(MUONX_PRIM_TRI, g, 9, current_xform(), mats.Get(mat.Record()), -1);
-      if (sc.compute_vertex_normals) {
-        // muon/parser.cc:630-637: accumulate the unnormalised face normal.
-        V3 n = cross(vs[1]->pos - vs[0]->pos, vs[2]->pos - vs[0]->pos);
-        verts[a].normal = verts[a].normal + n;
-        verts[b].normal = verts[b].normal + n;
-        verts[c].normal = verts[c].normal + n;
-        sc.prim_use_vnormal[id] = 1;
-      }
-      tri_refs.push_back(TriRef{id, cur_mesh, {a, b, c}});
+      if (iss.fail() || !add_tri(a, b, c)) { bad(); continue; }
     } else if (cmd == "translate" || cmd == "scale") {
       float x, y, z;
       iss >> x >> y >> z;
@@ -354,6 +530,7 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
       add_prim(MUONX_PRIM_TRI, t0, 9, xf, m, light_id);
       add_prim(MUONX_PRIM_TRI, t1, 9, xf, m, light_id);
     } else if (cmd == "brdf") {
+      mat_id = -1;
       std::string v;
       iss >> v;
       if (iss.fail()) { bad(); continue; }
@@ -361,6 +538,7 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
       else if (v == "ggx") mat.brdf = MUONX_BRDF_GGX;
       else bad();
     } else if (cmd == "ambient" || cmd == "diffuse" || cmd == "specular" || cmd == "emission") {
+      mat_id = -1;
       float r, g, b;
       iss >> r >> g >> b;
       if (iss.fail()) { bad(); continue; }
@@ -368,6 +546,7 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
                  : cmd == "specular" ? mat.specular : mat.emission;
       dst[0] = r; dst[1] = g; dst[2] = b;
     } else if (cmd == "shininess" || cmd == "roughness") {
+      mat_id = -1;
       float v;
       iss >> v;
       if (iss.fail()) { bad(); continue; }

@@ -103,6 +103,14 @@ def test_icosphere_workload_ingest_small(oracle):
     assert h.desc.num_prims == 20 * 4 ** 3 + 10 + 2
 
 
+def test_icosphere_workload_ingest_threaded_build(oracle):
+    """At 327,692 primitives the host build splits large subtrees over
+    threads (and parses with the fast number reader): the tree, the leaf
+    order and every array stay the sequential oracle's."""
+    text = scenes.icosphere_text(level=7)
+    _compare(HostScene.from_text(text), oracle.load_text(text))
+
+
 def test_icosphere_faces_point_outward():
     v, f = scenes.icosphere(3)
     n = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
