@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <thread>
 
@@ -220,10 +221,24 @@ class Builder {
       // the two halves touch disjoint ranges of info_
       NodeBuf lbuf, rbuf;
       int lroot = 0, rroot = 0;
-      std::thread t([&] { rroot = Build(rbuf, split, end, depth + 1, &rb); });
-      lroot = Build(lbuf, start, split, depth + 1, &lb);
+      std::exception_ptr err;  // an exception (allocation failure) reaches the caller's thread
+      std::thread t([&] {
+        try {
+          rroot = Build(rbuf, split, end, depth + 1, &rb);
+        } catch (...) {
+          err = std::current_exception();
+        }
+      });
+      try {
+        lroot = Build(lbuf, start, split, depth + 1, &lb);
+      } catch (...) {
+        t.join();
+        spare_.fetch_add(1);
+        throw;
+      }
       t.join();
       spare_.fetch_add(1);
+      if (err) std::rethrow_exception(err);
       l = buf.Append(lbuf) + lroot;
       r = buf.Append(rbuf) + rroot;
     } else {
@@ -287,7 +302,7 @@ void BuildBVH(HostScene &sc, int acceleration, int partition) {
       }
     };
     const size_t nt = n >= (size_t(1) << 16) ? hw : 1;
-    std::vector<std::thread> ts;
+    std::vector<std::thread> ts;  // WorldBox only reads the scene and writes its own slots
     for (size_t t = 1; t < nt; ++t) ts.emplace_back(boxes, n * t / nt, n * (t + 1) / nt);
     boxes(0, n / nt);
     for (auto &t : ts) t.join();

@@ -7,24 +7,33 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
-#include <map>
+#include <unordered_map>
 #include <sstream>
 
 namespace muonx {
 namespace host {
 namespace {
 
-// Builds one mesh: corners with the same attribute key share a vertex.
+// Builds one mesh: corners with the same attribute key share a vertex. Keys
+// are kept in one flat array and found through a hash of their words.
 class MeshBuilder {
  public:
   explicit MeshBuilder(std::vector<LoadedMesh> &out) : out_(out) {}
   // key: attribute bits identifying the corner; pos: its position
   int Corner(const std::vector<uint64_t> &key, const std::array<float, 3> &pos) {
-    auto it = index_.find(key);
-    if (it != index_.end()) return it->second;
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over the key words
+    for (uint64_t w : key) h = (h ^ w) * 1099511628211ull;
+    auto it = head_.find(h);
+    for (int v = it == head_.end() ? -1 : it->second; v >= 0; v = next_[size_t(v)]) {
+      const size_t o = off_[size_t(v)], len = off_[size_t(v) + 1] - o;
+      if (len == key.size() && std::equal(key.begin(), key.end(), keys_.begin() + long(o))) return v;
+    }
     const int id = int(cur_.pos.size());
     cur_.pos.push_back(pos);
-    index_.emplace(key, id);
+    keys_.insert(keys_.end(), key.begin(), key.end());
+    off_.push_back(keys_.size());
+    next_.push_back(it == head_.end() ? -1 : it->second);
+    head_[h] = id;
     return id;
   }
   // Adds a polygon as a fan from its first corner, dropping triangles with
@@ -40,13 +49,19 @@ class MeshBuilder {
   void Finish() {
     if (!cur_.tris.empty()) out_.push_back(std::move(cur_));
     cur_ = LoadedMesh();
-    index_.clear();
+    head_.clear();
+    keys_.clear();
+    off_.assign(1, 0);
+    next_.clear();
   }
 
  private:
   std::vector<LoadedMesh> &out_;
   LoadedMesh cur_;
-  std::map<std::vector<uint64_t>, int> index_;
+  std::unordered_map<uint64_t, int> head_;  // key hash -> newest vertex with it
+  std::vector<uint64_t> keys_;              // key words of every vertex
+  std::vector<size_t> off_{0};              // vertex v's key is keys_[off_[v], off_[v + 1])
+  std::vector<int> next_;                   // older vertex with the same hash, or -1
 };
 
 uint64_t Bits(float f) {
