@@ -2,6 +2,8 @@
 // in mesh_io.h.
 #include "mesh_io.h"
 
+#include "numbers.h"
+
 #include <algorithm>
 #include <cctype>
 #include <cstdint>
@@ -74,48 +76,130 @@ uint64_t Bits(float f) {
 // ------------------------------------------------------------------ OBJ
 
 // Resolves a 1-based (or negative, counted from the end) OBJ index.
-bool ObjIndex(const std::string &tok, size_t count, long &out) {
-  if (tok.empty()) return false;
-  char *end = nullptr;
-  const long v = std::strtol(tok.c_str(), &end, 10);
-  if (*end != '\0' || v == 0) return false;
+bool ObjResolve(long v, size_t count, long &out) {
+  if (v == 0) return false;
   const long idx = v > 0 ? v - 1 : long(count) + v;
   if (idx < 0 || size_t(idx) >= count) return false;
   out = idx;
   return true;
 }
 
+// Reads an optionally negative decimal integer of up to 9 digits at s.
+bool ObjInt(const char *&s, const char *e, long &out) {
+  bool neg = false;
+  if (s < e && *s == '-') {
+    neg = true;
+    ++s;
+  }
+  const char *b = s;
+  long v = 0;
+  while (s < e && *s >= '0' && *s <= '9' && s - b < 9) v = v * 10 + (*s++ - '0');
+  if (s == b || (s < e && *s >= '0' && *s <= '9')) return false;
+  out = neg ? -v : v;
+  return true;
+}
+
+// Splits the corners of an "f" line ("i", "i/j", "i//k", "i/j/k"; 0 marks
+// an absent index); false for anything else, which the stream path reads.
+bool ObjFaceFast(const char *s, const char *e, std::vector<std::array<long, 3>> &out) {
+  out.clear();
+  while (true) {
+    while (s < e && IsSpace(*s)) ++s;
+    if (s == e) return true;
+    std::array<long, 3> c{0, 0, 0};
+    if (!ObjInt(s, e, c[0])) return false;
+    if (s < e && *s == '/') {
+      ++s;
+      if (s < e && *s != '/' && !ObjInt(s, e, c[1])) return false;
+      if (s < e && *s == '/') {
+        ++s;
+        if (!ObjInt(s, e, c[2])) return false;
+      }
+    }
+    if (s < e && !IsSpace(*s)) return false;
+    out.push_back(c);
+  }
+}
+
 bool ParseObj(const std::string &data, std::vector<LoadedMesh> &out, std::string &err) {
   std::vector<std::array<float, 3>> v, vn;
   std::vector<std::array<float, 3>> vt;
   MeshBuilder mb(out);
-  std::istringstream in(data);
-  std::string line;
   size_t lineno = 0;
-  while (std::getline(in, line)) {
+  std::vector<int> corners;
+  std::vector<std::array<long, 3>> raw;
+  std::vector<uint64_t> key;
+  // one face corner from its raw OBJ indices (0: absent)
+  auto corner = [&](long rv, long rt, long rn) {
+    long iv, it = -1, in_ = -1;
+    if (!ObjResolve(rv, v.size(), iv) || (rt != 0 && !ObjResolve(rt, vt.size(), it)) ||
+        (rn != 0 && !ObjResolve(rn, vn.size(), in_)))
+      return false;
+    key.clear();
+    for (float c : v[size_t(iv)]) key.push_back(Bits(c));
+    key.push_back(it >= 0 ? 1u : 0u);
+    if (it >= 0)
+      for (float c : vt[size_t(it)]) key.push_back(Bits(c));
+    key.push_back(in_ >= 0 ? 1u : 0u);
+    if (in_ >= 0)
+      for (float c : vn[size_t(in_)]) key.push_back(Bits(c));
+    corners.push_back(mb.Corner(key, v[size_t(iv)]));
+    return true;
+  };
+  auto face = [&]() {
+    if (corners.size() < 3) {
+      err = "obj line " + std::to_string(lineno) + ": face with fewer than 3 corners";
+      return false;
+    }
+    mb.Polygon(corners);
+    return true;
+  };
+  for (size_t next = 0; next < data.size();) {
+    size_t lb = next, le = data.find('\n', next);
+    if (le == std::string::npos) le = data.size();
+    next = le + 1;
     ++lineno;
-    const size_t hash = line.find('#');
-    if (hash != std::string::npos) line.resize(hash);
-    std::istringstream ls(line);
+    const char *p = data.data() + lb, *pe = data.data() + le;
+    if (const void *h = std::memchr(p, '#', size_t(pe - p))) pe = static_cast<const char *>(h);
+    while (p < pe && IsSpace(*p)) ++p;
+    if (p == pe) continue;
+    // plain "v x y z" and "f ..." lines without a stream
+    if (pe - p > 2 && p[0] == 'v' && IsSpace(p[1])) {
+      float x[3];
+      if (FastNumbers(p + 2, pe, x, 3)) {
+        v.push_back({x[0], x[1], x[2]});
+        continue;
+      }
+    } else if (pe - p > 2 && p[0] == 'f' && IsSpace(p[1]) && ObjFaceFast(p + 2, pe, raw)) {
+      corners.clear();
+      for (const auto &c : raw)
+        if (!corner(c[0], c[1], c[2])) {
+          err = "obj line " + std::to_string(lineno) + ": bad face index";
+          return false;
+        }
+      if (!face()) return false;
+      continue;
+    }
+    std::istringstream ls(std::string(p, pe));
     std::string cmd;
     if (!(ls >> cmd)) continue;
     if (cmd == "v" || cmd == "vn") {
-      std::array<float, 3> p{};
-      if (!(ls >> p[0] >> p[1] >> p[2])) {
+      std::array<float, 3> q{};
+      if (!(ls >> q[0] >> q[1] >> q[2])) {
         err = "obj line " + std::to_string(lineno) + ": bad " + cmd;
         return false;
       }
-      (cmd == "v" ? v : vn).push_back(p);
+      (cmd == "v" ? v : vn).push_back(q);
     } else if (cmd == "vt") {
-      std::array<float, 3> p{};
-      if (!(ls >> p[0])) {
+      std::array<float, 3> q{};
+      if (!(ls >> q[0])) {
         err = "obj line " + std::to_string(lineno) + ": bad vt";
         return false;
       }
-      ls >> p[1] >> p[2];  // optional components
-      vt.push_back(p);
+      ls >> q[1] >> q[2];  // optional components
+      vt.push_back(q);
     } else if (cmd == "f") {
-      std::vector<int> corners;
+      corners.clear();
       std::string tok;
       while (ls >> tok) {
         std::string part[3];
@@ -126,27 +210,18 @@ bool ParseObj(const std::string &data, std::vector<LoadedMesh> &out, std::string
           part[1] = tok.substr(s0 + 1, s1 == std::string::npos ? std::string::npos : s1 - s0 - 1);
           if (s1 != std::string::npos) part[2] = tok.substr(s1 + 1);
         }
-        long iv, it = -1, in_ = -1;
-        if (!ObjIndex(part[0], v.size(), iv) || (!part[1].empty() && !ObjIndex(part[1], vt.size(), it)) ||
-            (!part[2].empty() && !ObjIndex(part[2], vn.size(), in_))) {
+        long iv, it = 0, in_ = 0;
+        char *end = nullptr;
+        bool ok = !part[0].empty();
+        if (ok) iv = std::strtol(part[0].c_str(), &end, 10), ok = *end == '\0';
+        if (ok && !part[1].empty()) it = std::strtol(part[1].c_str(), &end, 10), ok = *end == '\0' && it != 0;
+        if (ok && !part[2].empty()) in_ = std::strtol(part[2].c_str(), &end, 10), ok = *end == '\0' && in_ != 0;
+        if (!ok || !corner(iv, it, in_)) {
           err = "obj line " + std::to_string(lineno) + ": bad face index '" + tok + "'";
           return false;
         }
-        std::vector<uint64_t> key;
-        for (float c : v[size_t(iv)]) key.push_back(Bits(c));
-        key.push_back(it >= 0 ? 1u : 0u);
-        if (it >= 0)
-          for (float c : vt[size_t(it)]) key.push_back(Bits(c));
-        key.push_back(in_ >= 0 ? 1u : 0u);
-        if (in_ >= 0)
-          for (float c : vn[size_t(in_)]) key.push_back(Bits(c));
-        corners.push_back(mb.Corner(key, v[size_t(iv)]));
       }
-      if (corners.size() < 3) {
-        err = "obj line " + std::to_string(lineno) + ": face with fewer than 3 corners";
-        return false;
-      }
-      mb.Polygon(corners);
+      if (!face()) return false;
     } else if (cmd == "o" || cmd == "g" || cmd == "usemtl") {
       mb.Finish();
     }

@@ -213,3 +213,16 @@ def test_obj_groups_are_separate_meshes(tmp_path):
     vn = a["prim_vnormal"].reshape(2, 3, 3)
     assert np.array_equal(vn[0, 0], np.float32([0, 0, 1]))   # group a: only its face
     assert np.array_equal(vn[1, 0], np.float32([1, 0, 0]))   # group b: only its face
+
+
+def test_obj_face_forms_read_alike(tmp_path):
+    """Face lines taken by the fast reader and by the stream reader (an empty
+    texture index after a slash, a 10-digit index) give the same triangles."""
+    obj = ("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0.5 0.5\n"
+           "f 1 2 3\nf 1/ 2/ 3/\nf 0000000001 2 0000000003\nf -3/1 -2/1 -1/1\n")
+    (tmp_path / "m.obj").write_text(obj)
+    (tmp_path / "s.muon").write_text("load m.obj\n")
+    hs = HostScene.from_file(str(tmp_path / "s.muon"))
+    assert hs.warnings == 0 and hs.desc.num_prims == 4
+    g = hs.arrays()["prim_geom"][:, :9]
+    assert all(np.array_equal(g[0], g[k]) for k in range(1, 4))
