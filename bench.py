@@ -41,10 +41,12 @@ with the library's default configuration):
   launch, tail and synchronisation costs dominate, is reported as
   batched.small_jobs.
 * serving: the request loop under a Poisson stream of requests (resolution
-  {256^2, 512^2, 1024^2}, spp 8..128, random scene and pose) at a fixed rate chosen
-  as a load fraction of the batched throughput; images/s, p50/p99 latency
-  (submit to film written) and relMSE of a few requests against a
-  high-spp render of the same request.
+  {256^2, 512^2, 1024^2}, spp 8..128, random scene and pose) over its own 8
+  resident scenes (icospheres of 20k..1.3M tris, four of them at 1.3M, and
+  a sphere field: about 5.7M primitives) at a fixed rate chosen as a load
+  fraction of the batched throughput of the same request mix on those
+  scenes; images/s, p50/p99 latency (submit to film written) and relMSE of
+  a few requests against a high-spp render of the same request.
 """
 import argparse
 import json
@@ -66,9 +68,10 @@ def log(msg):
 def build_pool(ctx, log_fn, kinds=None):
     from muon_amd import DeviceScene, HostScene, scenes
     pool = []
-    for kind, size in (kinds or scenes.POOL):
+    for spec in (kinds or scenes.POOL):
+        kind, size, seed = (tuple(spec) + (0,))[:3]
         t0 = time.perf_counter()
-        hs = HostScene.from_text(scenes.pool_scene_text(kind, size))
+        hs = HostScene.from_text(scenes.pool_scene_text(kind, size, seed))
         ds = DeviceScene(ctx, hs)
         pool.append((kind, size, hs, ds, ds.info()))
         log_fn(f"bench: pool scene {kind} {size}: {ds.info()['num_prims']} prims, {time.perf_counter() - t0:.1f} s")
@@ -87,7 +90,8 @@ def make_job(pool, rng, res_choices, spp_lo, spp_hi, depth=8):
     return ds, p
 
 
-def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 1024), spp_range=(4, 64)):
+def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 1024), spp_range=(4, 64),
+                  sequential=True):
     import numpy as np
     import torch
     from muon_amd.renderer import render_device
@@ -107,6 +111,8 @@ def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 10
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     rays = st.primary_rays + st.secondary_rays + st.shadow_rays
+    if not sequential:
+        return None, samples / tb
     log_fn(f"bench: batched {tb:.2f} s; sequential plain renders")
     render_device(jobs[0][0], jobs[0][1])  # the plain path's engine
     torch.cuda.synchronize()
@@ -382,10 +388,21 @@ def main():
         log(f"bench: batched small jobs {small}")
         batched["small_jobs"] = {k: small[k] for k in ("jobs", "samples", "batched_s", "sequential_s", "images_per_s",
                                                         "sequential_images_per_s", "speedup", "identical_to_plain")}
-        serving = bench_serving(ctx, pool, args.serve_requests, cap, args.serve_load, 7, log)
-        log(f"bench: serving {serving}")
         pool_desc = [f"{k}:{i['num_prims']}" for k, _, _, _, i in pool]
         batched["scenes"] = pool_desc
+        del pool
+        # serving: its own 8 resident scenes (about 5.7M primitives); the
+        # request rate is a load fraction of the batched throughput measured
+        # on these scenes with the request mix
+        spool = build_pool(ctx, log, scenes.SERVE_POOL)
+        _, cap_serve = bench_batched(ctx, spool, 24, 9, log, res_choices=(256, 512, 1024), spp_range=(8, 128),
+                                     sequential=False)
+        serving = bench_serving(ctx, spool, args.serve_requests, cap_serve, args.serve_load, 7, log)
+        serving["capacity_msamples_per_s"] = round(cap_serve / 1e6, 1)
+        serving["scenes"] = [f"{k}:{i['num_prims']}" for k, _, _, _, i in spool]
+        serving["total_prims"] = int(sum(i["num_prims"] for _, _, _, _, i in spool))
+        log(f"bench: serving {serving}")
+        del spool
 
     out = {
         "metric": METRIC, "value": round(mrays, 1), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,

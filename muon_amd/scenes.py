@@ -277,11 +277,17 @@ def set_camera(params, eye, look, up=(0.0, 1.0, 0.0), fov_deg=None, width=None, 
     return params
 
 
-# Resident scenes for the batched and served workloads: displaced icospheres
-# of levels 3..8 (1,280 .. 1,310,720 tris) in the Cornell enclosure and
-# sphere fields, spanning ~1k to ~1.3M primitives.
+# Resident scenes for the batched workload: displaced icospheres of levels
+# 3..8 (1,280 .. 1,310,720 tris) in the Cornell enclosure and sphere fields,
+# spanning ~1k to ~1.3M primitives (sizes spread log-uniformly).
 POOL = [("ico", 3), ("ico", 4), ("ico", 5), ("ico", 6), ("ico", 7), ("ico", 8), ("spheres", 3000),
         ("spheres", 30000)]
+
+# Resident scenes for the served workload: 8 scenes with sizes in the same
+# range, about 5.7M primitives in all (four level-8 icospheres with different
+# displacement seeds); the third entry is the seed.
+SERVE_POOL = [("ico", 5, 0), ("ico", 6, 0), ("ico", 7, 0), ("ico", 8, 0), ("ico", 8, 1), ("ico", 8, 2),
+              ("ico", 8, 3), ("spheres", 30000, 0)]
 
 
 def pool_scene_text(kind, size, seed=0):
