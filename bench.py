@@ -216,7 +216,7 @@ def main():
                     help="run a 256 MiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

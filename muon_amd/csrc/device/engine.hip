@@ -1,4 +1,4 @@
-// Persistent wavefront engine (see engine.h): kernels and host driver.
+// Persistent wavefront engine (see engine.h): kernels and their host-side control.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -660,7 +660,7 @@ __global__ void k_film_resolve(const unsigned long long *__restrict__ film, uint
     out[i] = float(double(film[i]) / kFilmScale);
 }
 
-// ---------------------------------------------------------------- host driver
+// ---------------------------------------------------------------- host side
 namespace {
 
 // Sub-allocates `count` T at address `p` (256-byte aligned) and advances

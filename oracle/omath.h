@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only. Nothing in the muon_amd product links,
 // imports or executes this code; only tests/, __graft_entry__.smoke() and the
-// CPU-baseline leg of bench.py use it, as the checker.
+// CPU-baseline leg of bench.py use it, as the oracle results are checked against.
 //
 // Scalar FP32 vector / matrix math restated in the operation order of the
 // glm 0.9.9 routines the reference calls (no SIMD, no FMA contraction; the

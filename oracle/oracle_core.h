@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only (see omath.h). A CPU restatement of the
 // reference renderer (voithos/muon) kept op-for-op with its C++ sources so
-// that it can serve as the parity checker and the timed CPU baseline. It
+// that it can serve as the parity oracle and the timed CPU baseline. It
 // keeps the reference's algorithmic structure: heap-allocated binary BVH
 // nodes, per-primitive object-space transforms, optional-style results and
 // recursive integrators.

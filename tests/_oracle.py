@@ -1,6 +1,6 @@
 """ctypes bindings for oracle/liboracle.so, the CPU restatement of the
 reference renderer. Test infrastructure: used only by tests/, smoke() and
-bench.py's CPU baseline, always as the checker."""
+bench.py's CPU baseline, always as the oracle results are compared with."""
 import ctypes
 import os
 import subprocess

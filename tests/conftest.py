@@ -19,7 +19,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def oracle():
-    """The CPU restatement of the reference (the checker)."""
+    """The oracle: the CPU restatement of the reference."""
     import _oracle
     return _oracle.Oracle()
 
