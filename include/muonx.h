@@ -136,6 +136,14 @@ typedef struct muonx_trace_stats {
   uint64_t culled_rays; /* of secondary_rays, MIS rays resolved without a BVH traversal: their
                            direction misses every emitting primitive, so their closest hit
                            cannot be a light (path tracer; 0 for the other integrators) */
+  /* TraceStats' test counters (muon/stats.h:11-45, counted at
+   * muon/acceleration.cc:92-114,162-179): counted by the deterministic
+   * integrators, whose exact traversal visits the reference's nodes, so they
+   * equal the reference's; 0 for the path tracer, which walks a 4-wide tree. */
+  uint64_t object_tests;
+  uint64_t object_hits;
+  uint64_t bounds_tests;
+  uint64_t bounds_hits;
 } muonx_trace_stats;
 
 typedef struct muonx_scene_info {

@@ -45,7 +45,8 @@ class RenderParams(C.Structure):
 
 class TraceStatsC(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("secondary_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
-                ("render_ns", C.c_uint64), ("culled_rays", C.c_uint64)]
+                ("render_ns", C.c_uint64), ("culled_rays", C.c_uint64), ("object_tests", C.c_uint64),
+                ("object_hits", C.c_uint64), ("bounds_tests", C.c_uint64), ("bounds_hits", C.c_uint64)]
 
 
 class Query(C.Structure):

@@ -151,6 +151,7 @@ extern "C" int muonx_render_batch(muonx_ctx *ctx, const muonx_job *jobs, uint32_
     stats->shadow_rays = rays[2];
     stats->culled_rays = rays[3];
     stats->render_ns = NowNs() - t0;
+    stats->object_tests = stats->object_hits = stats->bounds_tests = stats->bounds_hits = 0;
   }
   return MUONX_OK;
 }

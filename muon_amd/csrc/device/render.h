@@ -15,6 +15,10 @@ constexpr int kSimpleBlock = 128;
 constexpr int kCtrPrimary = 0;
 constexpr int kCtrSecondary = 1;
 constexpr int kCtrShadow = 2;
+constexpr int kCtrObjectTests = 3;  // the reference's TraceStats test counters
+constexpr int kCtrObjectHits = 4;
+constexpr int kCtrBoundsTests = 5;
+constexpr int kCtrBoundsHits = 6;
 constexpr int kNumCounters = 8;
 
 struct CamParams {

@@ -44,11 +44,12 @@ CamParams MakeCam(const muonx_render_params &p) {
 template <int STACK>
 __global__ void __launch_bounds__(kSimpleBlock) k_simple(DevSceneView s, CamParams cam, int integrator, int spp,
                                                          int depth0_ok, uint32_t seed, float *__restrict__ accum,
-                                                         unsigned long long *__restrict__ rays) {
+                                                         unsigned long long *__restrict__ ctr) {
   __shared__ uint2 stack[STACK * kSimpleBlock];
   const uint32_t pixel = blockIdx.x * kSimpleBlock + threadIdx.x;
   const uint32_t npix = uint32_t(cam.width) * uint32_t(cam.height);
   uint32_t traced = 0;
+  TraceCounters cnt;
   if (pixel < npix) {
     const int px = int(pixel % uint32_t(cam.width)), py = int(pixel / uint32_t(cam.width));
     V3 acc{0.0f, 0.0f, 0.0f};
@@ -59,9 +60,8 @@ __global__ void __launch_bounds__(kSimpleBlock) k_simple(DevSceneView s, CamPara
       cast_ray(cam, x, y, o, d);
       const TraceRay r = make_ray(o, d);
       float best = __builtin_inff();
-      TraceCounters cnt;
       ++traced;
-      const int32_t p = trace_bvh<STACK, false, false, true>(s, r, best, stack + threadIdx.x, kSimpleBlock, cnt);
+      const int32_t p = trace_bvh<STACK, false, true, true>(s, r, best, stack + threadIdx.x, kSimpleBlock, cnt);
       V3 c{0.0f, 0.0f, 0.0f};
       if (p >= 0) {
         if (integrator == MUONX_INTEGRATOR_ALBEDO) {
@@ -83,9 +83,11 @@ __global__ void __launch_bounds__(kSimpleBlock) k_simple(DevSceneView s, CamPara
     accum[3 * size_t(pixel) + 1] = acc.y;
     accum[3 * size_t(pixel) + 2] = acc.z;
   }
-  unsigned long long a = traced;
-  for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
-  if ((threadIdx.x & 63) == 0 && a) atomicAdd(rays, a);
+  wave_count(traced, &ctr[kCtrPrimary]);
+  wave_count(cnt.prims, &ctr[kCtrObjectTests]);
+  wave_count(cnt.ohits, &ctr[kCtrObjectHits]);
+  wave_count(cnt.btests, &ctr[kCtrBoundsTests]);
+  wave_count(cnt.bhits, &ctr[kCtrBoundsHits]);
 }
 
 // Raytracer (muon/integration.cc:78-120) and AnalyticDirect
@@ -124,6 +126,7 @@ __global__ void __launch_bounds__(kSimpleBlock) k_whitted(DevSceneView s, CamPar
   const uint32_t pixel = blockIdx.x * kSimpleBlock + threadIdx.x;
   const uint32_t npix = uint32_t(cam.width) * uint32_t(cam.height);
   uint32_t n_primary = 0, n_secondary = 0, n_shadow = 0;
+  TraceCounters cnt;
   if (pixel < npix) {
     const int px = int(pixel % uint32_t(cam.width)), py = int(pixel / uint32_t(cam.width));
     V3 acc{0.0f, 0.0f, 0.0f};
@@ -143,8 +146,7 @@ __global__ void __launch_bounds__(kSimpleBlock) k_whitted(DevSceneView s, CamPar
         else ++n_secondary;
         const TraceRay r = make_ray(o, d);
         float best = __builtin_inff();
-        TraceCounters cnt;
-        const int32_t p = trace_bvh<STACK, false, false, true>(s, r, best, stack + threadIdx.x, kSimpleBlock, cnt);
+        const int32_t p = trace_bvh<STACK, false, true, true>(s, r, best, stack + threadIdx.x, kSimpleBlock, cnt);
         if (p < 0) break;
         const HitRecord h = hit_record(s, uint32_t(p), o, d);
         const float *m = s.materials + size_t(s.prim_mat[p]) * kMatStride;
@@ -160,8 +162,8 @@ __global__ void __launch_bounds__(kSimpleBlock) k_whitted(DevSceneView s, CamPar
             ++n_shadow;
             const TraceRay sr = make_ray(shift, ldir);
             float sbest = ldist;
-            const int32_t occ = trace_bvh<STACK, true, false, true>(s, sr, sbest, stack + threadIdx.x, kSimpleBlock,
-                                                                    cnt);
+            const int32_t occ = trace_bvh<STACK, true, true, true>(s, sr, sbest, stack + threadIdx.x, kSimpleBlock,
+                                                                   cnt);
             if (occ >= 0) continue;
             const V3 diffuse = kd * gmax(dot(h.normal, ldir), 0.0f);
             const V3 half = normalize(ldir - d);
@@ -216,9 +218,13 @@ __global__ void __launch_bounds__(kSimpleBlock) k_whitted(DevSceneView s, CamPar
     accum[3 * size_t(pixel) + 1] = acc.y;
     accum[3 * size_t(pixel) + 2] = acc.z;
   }
-  wave_count(n_primary, &rays[0]);
-  wave_count(n_secondary, &rays[1]);
-  wave_count(n_shadow, &rays[2]);
+  wave_count(n_primary, &rays[kCtrPrimary]);
+  wave_count(n_secondary, &rays[kCtrSecondary]);
+  wave_count(n_shadow, &rays[kCtrShadow]);
+  wave_count(cnt.prims, &rays[kCtrObjectTests]);
+  wave_count(cnt.ohits, &rays[kCtrObjectHits]);
+  wave_count(cnt.btests, &rays[kCtrBoundsTests]);
+  wave_count(cnt.bhits, &rays[kCtrBoundsHits]);
 }
 
 int RenderWhitted(muonx_scene *scene, const muonx_render_params &p, float *accum, unsigned long long *rays,
@@ -308,7 +314,7 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
     case MUONX_INTEGRATOR_NORMALS:
     case MUONX_INTEGRATOR_ALBEDO:
     case MUONX_INTEGRATOR_DEPTH:
-      rc = RenderSimple(scene, p, accum, ctr + kCtrPrimary, stream);
+      rc = RenderSimple(scene, p, accum, ctr, stream);
       break;
     case MUONX_INTEGRATOR_RAYTRACER:
     case MUONX_INTEGRATOR_ANALYTICDIRECT:
@@ -339,6 +345,10 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
     stats->shadow_rays = host_rays ? rays[2] : h[kCtrShadow];
     stats->render_ns = uint64_t(double(ms) * 1e6);
     stats->culled_rays = host_rays ? rays[3] : 0;
+    stats->object_tests = host_rays ? 0 : h[kCtrObjectTests];
+    stats->object_hits = host_rays ? 0 : h[kCtrObjectHits];
+    stats->bounds_tests = host_rays ? 0 : h[kCtrBoundsTests];
+    stats->bounds_hits = host_rays ? 0 : h[kCtrBoundsHits];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

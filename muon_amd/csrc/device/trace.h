@@ -142,8 +142,11 @@ __device__ __forceinline__ void to_object(const float *xf, V3 o, V3 d, V3 &oo, V
 }
 
 struct TraceCounters {
-  uint32_t nodes;  // internal nodes fetched (two child boxes each)
-  uint32_t prims;  // primitive tests
+  uint32_t nodes = 0;   // internal nodes fetched (two child boxes each)
+  uint32_t prims = 0;   // primitive tests (the reference's object tests)
+  uint32_t ohits = 0;   // object hits (exact mode, as muon/acceleration.cc:108-114,177-179)
+  uint32_t btests = 0;  // bounds tests (exact mode, as muon/acceleration.cc:92,162)
+  uint32_t bhits = 0;   // bounds hits (exact mode, as muon/acceleration.cc:101,171)
 };
 
 // Tests leaf-order primitive i against the ray; returns the world distance
@@ -306,6 +309,27 @@ __device__ __forceinline__ bool node_step(const DevSceneView &s, const TraceRay 
   const bool hn = swap ? h1 : h0;
   const bool hf = swap ? h0 : h1;
   const float lof = swap ? lo0 : lo1;
+  if (COUNT && EXACT) {
+    // The reference (muon/acceleration.cc:137,197) always pushes the far
+    // child and tests its box only when it pops it, which an any-hit query
+    // that stops at its first hit (:182-183) never does. So with the near box hit the
+    // far child is pushed too, with an infinite key when its box was missed
+    // here, and its test is counted at its pop (pop_next_counted).
+    if (hn) {
+      cnt.btests++;
+      cnt.bhits++;
+      stack.push(make_uint2(far_ref, hf ? __float_as_uint(lof) : 0x7f800000u));
+      ref = near_ref;
+      return true;
+    }
+    cnt.btests += 2;  // the near box, then the far one popped at once
+    if (hf) {
+      cnt.bhits++;
+      ref = far_ref;
+      return true;
+    }
+    return false;
+  }
   if (hn) {
     if (hf) stack.push(make_uint2(far_ref, __float_as_uint(lof)));
     ref = near_ref;
@@ -331,6 +355,23 @@ __device__ __forceinline__ bool pop_next(Stack &stack, float best, uint32_t &ref
   return false;
 }
 
+// pop_next for the counted exact traversal: every popped far child is the
+// reference's bounds test at its pop, a hit when its box still starts
+// before `best`.
+template <class Stack>
+__device__ __forceinline__ bool pop_next_counted(Stack &stack, float best, uint32_t &ref, TraceCounters &cnt) {
+  while (!stack.empty()) {
+    const uint2 e = stack.pop();
+    cnt.btests++;
+    if (__uint_as_float(e.y) < best) {
+      cnt.bhits++;
+      ref = e.x;
+      return true;
+    }
+  }
+  return false;
+}
+
 // Tests the primitives of the leaf at `ref`; returns true if ANY found a hit.
 // A wide-tree leaf (WIDE) whose reference carries a count tests that many
 // primitives; otherwise the leaf ends at the primitive flagged last.
@@ -346,7 +387,11 @@ __device__ __forceinline__ bool leaf_step(const DevSceneView &s, const TraceRay 
     float dist;
     uint32_t flags;
     if (COUNT) cnt.prims++;
-    if (prim_hit(s, i, r, oc, dist, flags) && dist > 0.0f && dist < best) {
+    const bool ph = prim_hit(s, i, r, oc, dist, flags);
+    // an object hit is Primitive::Intersect's (closest hit) or
+    // HasIntersection's (any hit: also 0 < dist < tmax), muon/objects.cc:9-17
+    if (COUNT && ph && (!ANY || (dist > 0.0f && dist < best))) cnt.ohits++;
+    if (ph && dist > 0.0f && dist < best) {
       best = dist;
       hit_prim = int32_t(i);
       if (ANY) return true;
@@ -371,18 +416,38 @@ __device__ __forceinline__ int32_t trace_bvh_s(const DevSceneView &s, const Trac
   ObjCache oc;
   oc.tid = -1;
   uint32_t ref = 0;  // root: internal node 0
+  constexpr bool kRefStats = COUNT && EXACT;
+  if (kRefStats) {
+    // The reference tests the root's box first (muon/acceleration.cc:92-99);
+    // here it is the union of node 0's child boxes. A ray that misses it
+    // misses every box below, so stopping changes no result. A root that is
+    // a leaf sits in node 0's first child (the second is a sentinel).
+    const float4 *np = s.nodes;
+    const float4 n0 = np[0], n1 = np[1], n2 = np[2];
+    float lo;
+    cnt.btests++;
+    if (!box_hit_exact(r, fminf(n0.x, n1.x), fmaxf(n0.y, n1.y), fminf(n0.z, n1.z), fmaxf(n0.w, n1.w),
+                       fminf(n2.x, n2.z), fmaxf(n2.y, n2.w), best, lo))
+      return -1;
+    cnt.bhits++;
+    if (s.root_leaf) ref = __float_as_uint(np[3].x);
+  }
+  auto pop = [&](uint32_t &rf) {
+    if constexpr (kRefStats) return pop_next_counted(stack, best, rf, cnt);
+    else return pop_next(stack, best, rf);
+  };
   if (WW) {
     while (true) {
       bool alive = true;
       while (!(ref & kLeafBit)) {
-        if (!node_step<COUNT, EXACT>(s, r, best, stack, ref, cnt) && !pop_next(stack, best, ref)) {
+        if (!node_step<COUNT, EXACT>(s, r, best, stack, ref, cnt) && !pop(ref)) {
           alive = false;
           break;
         }
       }
       if (!alive) break;
       if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
-      if (!pop_next(stack, best, ref)) break;
+      if (!pop(ref)) break;
     }
   } else {
     while (true) {
@@ -391,7 +456,7 @@ __device__ __forceinline__ int32_t trace_bvh_s(const DevSceneView &s, const Trac
       } else {
         if (leaf_step<ANY, COUNT>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
       }
-      if (!pop_next(stack, best, ref)) break;
+      if (!pop(ref)) break;
     }
   }
   return hit_prim;

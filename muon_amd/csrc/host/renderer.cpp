@@ -69,6 +69,14 @@ int Renderer::Render() const {
     std::cerr << "Primary rays       : " << std::setw(12) << st.primary_rays << "\n";
     std::cerr << "Secondary rays     : " << std::setw(12) << st.secondary_rays << "\n";
     std::cerr << "Shadow rays        : " << std::setw(12) << st.shadow_rays << "\n";
+    // muon/stats.cc:41-88 (counted by the deterministic integrators)
+    auto rate = [](uint64_t h, uint64_t t) { return t ? 100.0 * double(h) / double(t) : 0.0; };
+    std::cerr << "Ray-object tests   : " << std::setw(12) << st.object_tests << "\n";
+    std::cerr << "Ray-object hits    : " << std::setw(12) << st.object_hits << "\n";
+    std::cerr << "Hit rate           : " << std::setw(12) << rate(st.object_hits, st.object_tests) << " %\n";
+    std::cerr << "Bounds tests       : " << std::setw(12) << st.bounds_tests << "\n";
+    std::cerr << "Bounds hits        : " << std::setw(12) << st.bounds_hits << "\n";
+    std::cerr << "Bounds hit rate    : " << std::setw(12) << rate(st.bounds_hits, st.bounds_tests) << " %\n";
   }
   muonx_scene_destroy(scene);
   muonx_ctx_destroy(ctx);

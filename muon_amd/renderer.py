@@ -38,14 +38,28 @@ class TraceStats:
     render_seconds: float = 0.0
     build_seconds: float = 0.0
     culled_rays: int = 0  # of secondary_rays: MIS rays settled without a traversal
+    # muon/stats.h:11-45 test counters (deterministic integrators; 0 for the path tracer)
+    object_tests: int = 0
+    object_hits: int = 0
+    bounds_tests: int = 0
+    bounds_hits: int = 0
 
     def __str__(self):
+        # muon/stats.cc:41-88
+        def rate(h, t):
+            return 100.0 * h / t if t else 0.0
         lines = [
             f"{'Build time':18s} : {self.build_seconds:12.2f} (sec)",
             f"{'Render time':18s} : {self.render_seconds:12.2f} (sec)",
             f"{'Primary rays':18s} : {self.primary_rays:12d}",
             f"{'Secondary rays':18s} : {self.secondary_rays:12d}",
             f"{'Shadow rays':18s} : {self.shadow_rays:12d}",
+            f"{'Ray-object tests':18s} : {self.object_tests:12d}",
+            f"{'Ray-object hits':18s} : {self.object_hits:12d}",
+            f"{'Hit rate':18s} : {rate(self.object_hits, self.object_tests):12.2f} %",
+            f"{'Bounds tests':18s} : {self.bounds_tests:12d}",
+            f"{'Bounds hits':18s} : {self.bounds_hits:12d}",
+            f"{'Bounds hit rate':18s} : {rate(self.bounds_hits, self.bounds_tests):12.2f} %",
         ]
         return "\n".join(lines)
 
@@ -88,7 +102,8 @@ def render_device(dev_scene, params, accum=None):
     s = torch.cuda.current_stream(accum.device).cuda_stream
     check(lib.muonx_render(dev_scene.h, C.byref(params), accum.data_ptr(), C.byref(st), s))
     stats = TraceStats(st.primary_rays, st.secondary_rays, st.shadow_rays, st.render_ns * 1e-9,
-                       culled_rays=st.culled_rays)
+                       culled_rays=st.culled_rays, object_tests=st.object_tests, object_hits=st.object_hits,
+                       bounds_tests=st.bounds_tests, bounds_hits=st.bounds_hits)
     return accum, stats
 
 

@@ -61,6 +61,35 @@ def test_deterministic_film_matches_oracle_and_golden(ctx, oracle, ref_dir, name
     assert stats.primary_rays == ref_st[0]
     if hs.params.integrator == 3:  # raytracer: reflection and shadow rays as the reference traces them
         assert stats.secondary_rays == ref_st[1] and stats.shadow_rays == ref_st[6]
+    if hs.params.integrator != 5:
+        # TraceStats' object / bounds tests and hits (muon/acceleration.cc:92-114,
+        # 162-179): the exact traversal visits the reference's nodes
+        assert _counters(stats) == tuple(int(x) for x in ref_st[2:6])
+
+
+def _counters(stats):
+    return (stats.object_tests, stats.object_hits, stats.bounds_tests, stats.bounds_hits)
+
+
+@pytest.mark.parametrize("scene", ["one sphere", "two tris", "cornell"])
+def test_trace_counters_match_oracle(ctx, oracle, scene):
+    """The reference's test counters on scenes whose BVH root is a leaf (one
+    sphere; two triangles the SAH keeps in one leaf) and on the Cornell box
+    with the depth integrator (one sample per pixel: later samples are
+    jittered by streams that differ between the GPU and the CPU)."""
+    if scene == "cornell":
+        text = scenes.cornell_text(48, 40, 1, 5).replace("integrator pathtracer", "integrator depth")
+    else:
+        text = "film_size 40 30\ncamera 0 0 4 0 0 0 0 1 0 45\nintegrator normals\n"
+        text += ("sphere 0.2 0 -1 0.9\n" if scene == "one sphere" else
+                 "vertex -1 -1 0\nvertex 1 -1 0\nvertex 1 1 0\ntri 0 1 2\nvertex -1 1 0\ntri 0 2 3\n")
+    hs = HostScene.from_text(text, default_seed=5)
+    acc, stats = gpu_render(ctx, hs)
+    ref_acc, _, ref_st, _ = oracle.load_text(text, seed=5).render(1)
+    assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32))
+    assert stats.primary_rays == ref_st[0]
+    assert _counters(stats) == tuple(int(x) for x in ref_st[2:6])
+    assert stats.bounds_tests > 0 and stats.object_tests > 0
 
 
 def test_dragon_vertex_normals_match_oracle(ctx, oracle, ref_dir):
@@ -76,6 +105,7 @@ def test_dragon_vertex_normals_match_oracle(ctx, oracle, ref_dir):
     assert (np.abs(acc - ref_acc) <= 1e-5 * np.maximum(np.abs(ref_acc), 1.0)).all()
     assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32))
     assert (acc.sum(axis=2) > 0).mean() > 0.3
+    assert _counters(stats) == tuple(int(x) for x in ref_st[2:6])
 
 
 def test_albedo_integrator(ctx, oracle, ref_dir):
