@@ -102,6 +102,9 @@ typedef struct muonx_scene_desc {
   const int32_t *node_start;       /* leaf: first index into leaf_prims */
   const int32_t *node_count;       /* leaf: primitive count; 0 for internal nodes */
   const int32_t *leaf_prims;       /* num_prims parse-order ids in leaf order */
+  int32_t acceleration;            /* 1: BVH; 0: the linear structure (muon/acceleration.cc:14-45),
+                                      one leaf in parse order, whose queries count tests as
+                                      Linear does */
 } muonx_scene_desc;
 
 /* Per-render settings: the scene's render properties (muon/scene.h:31-51)

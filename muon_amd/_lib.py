@@ -27,7 +27,7 @@ class SceneDesc(C.Structure):
         ("num_lights", C.c_uint32), ("lights", C.c_void_p),
         ("num_nodes", C.c_uint32), ("node_bounds", C.c_void_p), ("node_left", C.c_void_p),
         ("node_right", C.c_void_p), ("node_axis", C.c_void_p), ("node_start", C.c_void_p),
-        ("node_count", C.c_void_p), ("leaf_prims", C.c_void_p),
+        ("node_count", C.c_void_p), ("leaf_prims", C.c_void_p), ("acceleration", C.c_int32),
     ]
 
 

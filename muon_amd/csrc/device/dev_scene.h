@@ -86,6 +86,7 @@ struct DevSceneView {
   uint32_t wstack;          // bound on stack entries of a wide traversal
   uint32_t leaf_counts;     // 1: wide leaf references carry their primitive count
   uint32_t root_leaf;       // 1: the BVH root is a leaf (node 0 holds it and a sentinel leaf)
+  uint32_t linear;          // 1: the linear structure (one leaf in parse order)
   // Emitting primitives (prim_light >= 0), leaf order, when there are at
   // most kMaxLightPrims of them (cull_mis = 1): a MIS ray that hits none of
   // them needs no traversal.

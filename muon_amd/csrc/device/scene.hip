@@ -487,6 +487,7 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   v.wstack = wstack;
   v.leaf_counts = leaf_counts ? 1u : 0u;
   v.root_leaf = root_leaf ? 1u : 0u;
+  v.linear = (d->acceleration == 0 && root_leaf) ? 1u : 0u;
   v.light_prims = s->light_prims.as<uint32_t>();
   v.cull_mis = cull_mis ? 1u : 0u;
   {

@@ -417,7 +417,16 @@ __device__ __forceinline__ int32_t trace_bvh_s(const DevSceneView &s, const Trac
   oc.tid = -1;
   uint32_t ref = 0;  // root: internal node 0
   constexpr bool kRefStats = COUNT && EXACT;
-  if (kRefStats) {
+  if (kRefStats && s.linear) {
+    // Linear::Intersect tests every primitive and counts object tests and
+    // hits only; Linear::HasIntersection counts nothing
+    // (muon/acceleration.cc:14-45). The one leaf is node 0's first child.
+    if constexpr (ANY) {
+      TraceCounters none;
+      return trace_bvh_s<ANY, false, EXACT, WW>(s, r, best, stack, none);
+    }
+    ref = __float_as_uint(s.nodes[3].x);
+  } else if (kRefStats) {
     // The reference tests the root's box first (muon/acceleration.cc:92-99);
     // here it is the union of node 0's child boxes. A ray that misses it
     // misses every box below, so stopping changes no result. A root that is

@@ -282,6 +282,7 @@ class Builder {
 
 void BuildBVH(HostScene &sc, int acceleration, int partition) {
   const size_t n = sc.num_prims();
+  sc.acceleration = acceleration == 0 ? 0 : 1;
   sc.node_bounds.clear();
   sc.node_left.clear();
   sc.node_right.clear();

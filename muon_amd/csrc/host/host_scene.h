@@ -49,6 +49,7 @@ struct HostScene {
   std::vector<int32_t> node_left, node_right, node_axis, node_start, node_count;
   std::vector<int32_t> leaf_prims;
   int bvh_depth = 0;
+  int acceleration = 1;  // 0: linear structure (one leaf in parse order)
 
   size_t num_prims() const { return prim_type.size(); }
   muonx_scene_desc Desc() const;

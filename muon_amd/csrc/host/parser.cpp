@@ -484,6 +484,7 @@ muonx_scene_desc HostScene::Desc() const {
   d.node_start = node_start.data();
   d.node_count = node_count.data();
   d.leaf_prims = leaf_prims.data();
+  d.acceleration = acceleration;
   return d;
 }
 

@@ -71,25 +71,36 @@ def _counters(stats):
     return (stats.object_tests, stats.object_hits, stats.bounds_tests, stats.bounds_hits)
 
 
-@pytest.mark.parametrize("scene", ["one sphere", "two tris", "cornell"])
-def test_trace_counters_match_oracle(ctx, oracle, scene):
+RAYTRACE_SMALL = ("film_size 40 30\ncamera 0 0.5 5 0 0 0 0 1 0 45\nintegrator raytracer\nmax_depth 3\n"
+                  "point_light 2 3 2 1 1 1\ndirectional_light -1 1 1 0.3 0.3 0.3\nspecular 0.3 0.3 0.3\n"
+                  "diffuse 0.5 0.2 0.1\nsphere 0 0 0 0.8\nsphere 1.2 0.3 -0.5 0.4\n"
+                  "vertex -3 -1 -3\nvertex 3 -1 -3\nvertex 3 -1 3\nvertex -3 -1 3\ntri 0 2 1\ntri 0 3 2\n")
+
+
+@pytest.mark.parametrize("scene,accel", [("one sphere", 1), ("two tris", 1), ("cornell", 1), ("cornell", 0),
+                                         ("raytrace", 1), ("raytrace", 0)])
+def test_trace_counters_match_oracle(ctx, oracle, scene, accel):
     """The reference's test counters on scenes whose BVH root is a leaf (one
-    sphere; two triangles the SAH keeps in one leaf) and on the Cornell box
+    sphere; two triangles the SAH keeps in one leaf), on the Cornell box
     with the depth integrator (one sample per pixel: later samples are
-    jittered by streams that differ between the GPU and the CPU)."""
+    jittered by streams that differ between the GPU and the CPU) and on a
+    raytracer scene with shadow rays, with the BVH and with the linear
+    structure (which counts object tests only, and nothing for shadow rays)."""
     if scene == "cornell":
         text = scenes.cornell_text(48, 40, 1, 5).replace("integrator pathtracer", "integrator depth")
+    elif scene == "raytrace":
+        text = RAYTRACE_SMALL
     else:
         text = "film_size 40 30\ncamera 0 0 4 0 0 0 0 1 0 45\nintegrator normals\n"
         text += ("sphere 0.2 0 -1 0.9\n" if scene == "one sphere" else
                  "vertex -1 -1 0\nvertex 1 -1 0\nvertex 1 1 0\ntri 0 1 2\nvertex -1 1 0\ntri 0 2 3\n")
-    hs = HostScene.from_text(text, default_seed=5)
+    hs = HostScene.from_text(text, acceleration=accel, default_seed=5)
     acc, stats = gpu_render(ctx, hs)
-    ref_acc, _, ref_st, _ = oracle.load_text(text, seed=5).render(1)
+    ref_acc, _, ref_st, _ = oracle.load_text(text, seed=5, accel=accel).render(1)
     assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32))
     assert stats.primary_rays == ref_st[0]
     assert _counters(stats) == tuple(int(x) for x in ref_st[2:6])
-    assert stats.bounds_tests > 0 and stats.object_tests > 0
+    assert stats.object_tests > 0 and (stats.bounds_tests > 0) == (accel == 1)
 
 
 def test_dragon_vertex_normals_match_oracle(ctx, oracle, ref_dir):
