@@ -32,7 +32,8 @@ def test_cli_matches_reference_golden(tmp_path, ref_dir, name):
     out = tmp_path / "out.png"
     r = _run(["--scene=%s" % (tmp_path / "scene.muon"), "--output=%s" % out, "--seed=3", "--stats=true"])
     assert r.returncode == 0, r.stderr
-    assert "Primary rays" in r.stderr
+    for label in ("Primary rays", "Ray-object tests", "Ray-object hits", "Bounds tests", "Bounds hits"):
+        assert label in r.stderr
     got, gold = _png(out), _png(os.path.join(ref_dir, name + ".png"))
     assert got.shape == gold.shape
     assert int((got != gold).any(axis=2).sum()) == 0

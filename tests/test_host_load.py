@@ -226,3 +226,33 @@ def test_obj_face_forms_read_alike(tmp_path):
     assert hs.warnings == 0 and hs.desc.num_prims == 4
     g = hs.arrays()["prim_geom"][:, :9]
     assert all(np.array_equal(g[0], g[k]) for k in range(1, 4))
+
+
+def test_ply_double_coordinates_and_wide_list_counts(tmp_path):
+    """Binary PLY with double coordinates, a ushort list count, uint indices
+    and a trailing element: read as the same triangles as inline."""
+    verts = [(0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (1.0, 1.0, 0.5), (0.0, 1.0, 0.25)]
+    head = ["ply", "format binary_big_endian 1.0", "element vertex 4", "property double x", "property double y",
+            "property double z", "element face 1", "property list ushort uint vertex_index", "element extra 1",
+            "property list uchar float values", "end_header"]
+    data = ("\n".join(head) + "\n").encode()
+    for p in verts:
+        data += struct.pack(">ddd", *p)
+    data += struct.pack(">HIIII", 4, 0, 1, 2, 3)
+    data += struct.pack(">Bff", 2, 1.0, 2.0)
+    (tmp_path / "q.ply").write_bytes(data)
+    (tmp_path / "s.muon").write_text(HEAD + "load q.ply\n" + TAIL)
+    hs = HostScene.from_file(str(tmp_path / "s.muon"))
+    ref = HostScene.from_text(HEAD + _expanded(verts, [(0, 1, 2), (0, 2, 3)]) + TAIL)
+    assert hs.warnings == 0
+    _same_arrays(hs, ref)
+
+
+def test_ply_truncated_body_is_a_warning(tmp_path):
+    data = b"ply\nformat binary_little_endian 1.0\nelement vertex 3\nproperty float x\nproperty float y\n" \
+           b"property float z\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n" + \
+           struct.pack("<fff", 0, 0, 0)
+    (tmp_path / "t.ply").write_bytes(data)
+    (tmp_path / "s.muon").write_text(HEAD + "load t.ply\n" + TAIL)
+    hs = HostScene.from_file(str(tmp_path / "s.muon"))
+    assert hs.warnings == 1 and hs.desc.num_prims == 1
