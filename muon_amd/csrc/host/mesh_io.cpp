@@ -379,9 +379,13 @@ bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string
     return false;
   }
   PlyReader rd(data, size_t(body), fmt);
-  std::vector<std::vector<uint64_t>> vkey;
+  // vertex keys (the bits of every scalar property, `vstride` per vertex)
+  // and face corner lists, flat
+  std::vector<uint64_t> vkey;
+  size_t vstride = 0;
   std::vector<std::array<float, 3>> vpos;
-  std::vector<std::vector<long>> faces;
+  std::vector<long> fidx;
+  std::vector<size_t> foff{0};
   for (const PlyElement &e : elems) {
     const bool is_vertex = e.name == "vertex", is_face = e.name == "face";
     int ix = -1, iy = -1, iz = -1, ifl = -1;
@@ -401,10 +405,12 @@ bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string
       err = "ply: face element without vertex_indices";
       return false;
     }
+    if (is_vertex) {
+      vstride = 0;
+      for (const PlyProp &p : e.props) vstride += p.list ? 0 : 1;
+    }
     for (size_t r = 0; r < e.count; ++r) {
-      std::vector<uint64_t> key;
       std::array<float, 3> pos{};
-      std::vector<long> face;
       for (size_t k = 0; k < e.props.size(); ++k) {
         const PlyProp &p = e.props[k];
         double v;
@@ -419,7 +425,7 @@ bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string
               err = "ply: truncated list";
               return false;
             }
-            if (is_face && int(k) == ifl) face.push_back(long(v));
+            if (is_face && int(k) == ifl) fidx.push_back(long(v));
           }
           continue;
         }
@@ -429,30 +435,32 @@ bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string
         }
         if (is_vertex) {
           const float f = float(v);
-          key.push_back(Bits(f));
+          vkey.push_back(Bits(f));
           if (int(k) == ix) pos[0] = f;
           else if (int(k) == iy) pos[1] = f;
           else if (int(k) == iz) pos[2] = f;
         }
       }
-      if (is_vertex) {
-        vkey.push_back(std::move(key));
-        vpos.push_back(pos);
-      } else if (is_face) {
-        faces.push_back(std::move(face));
-      }
+      if (is_vertex) vpos.push_back(pos);
+      else if (is_face) foff.push_back(fidx.size());
     }
   }
   MeshBuilder mb(out);
   std::vector<int> joined(vpos.size());
-  for (size_t i = 0; i < vpos.size(); ++i) joined[i] = mb.Corner(vkey[i], vpos[i]);
-  for (const auto &f : faces) {
-    if (f.size() < 3) {
+  std::vector<uint64_t> key(vstride);
+  for (size_t i = 0; i < vpos.size(); ++i) {
+    std::copy(vkey.begin() + long(i * vstride), vkey.begin() + long((i + 1) * vstride), key.begin());
+    joined[i] = mb.Corner(key, vpos[i]);
+  }
+  std::vector<int> c;
+  for (size_t fi = 0; fi + 1 < foff.size(); ++fi) {
+    if (foff[fi + 1] - foff[fi] < 3) {
       err = "ply: face with fewer than 3 corners";
       return false;
     }
-    std::vector<int> c;
-    for (long i : f) {
+    c.clear();
+    for (size_t j = foff[fi]; j < foff[fi + 1]; ++j) {
+      const long i = fidx[j];
       if (i < 0 || size_t(i) >= vpos.size()) {
         err = "ply: face index out of range";
         return false;
