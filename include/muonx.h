@@ -304,7 +304,13 @@ typedef struct muonx_engine_config {
   uint32_t iterations_per_poll; /* engine iterations between completion checks; 0: 2 */
   uint64_t film_budget_bytes;   /* HBM for in-flight films, 24 B per pixel; 0: 1 GiB */
   int32_t use_graph;            /* replay each iteration as a captured hipGraph */
+  int32_t slice_order;          /* which running render gets the next slice of samples:
+                                   MUONX_SLICES_OLDEST_FIRST (0) or
+                                   MUONX_SLICES_LEAST_WORK_FIRST (1: lower median latency
+                                   under load, longer tail) */
 } muonx_engine_config;
+#define MUONX_SLICES_OLDEST_FIRST 0
+#define MUONX_SLICES_LEAST_WORK_FIRST 1
 
 typedef struct muonx_job {
   muonx_scene *scene;            /* resident scene (its context = the engine's) */

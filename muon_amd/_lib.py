@@ -74,7 +74,8 @@ class SceneInfo(C.Structure):
 
 class EngineConfig(C.Structure):
     _fields_ = [("path_slots", C.c_uint32), ("max_jobs", C.c_uint32), ("shadow_per_vertex", C.c_int32),
-                ("iterations_per_poll", C.c_uint32), ("film_budget_bytes", C.c_uint64), ("use_graph", C.c_int32)]
+                ("iterations_per_poll", C.c_uint32), ("film_budget_bytes", C.c_uint64), ("use_graph", C.c_int32),
+                ("slice_order", C.c_int32)]
 
 
 class Job(C.Structure):

@@ -72,6 +72,10 @@ uint64_t PathSlotBytes(int kmax) { return 88 + 152 + 34 * uint64_t(kmax); }
 int Scheduler::Create(muonx_ctx *ctx, const muonx_tuning &tuning, const muonx_engine_config &in, Scheduler **out) {
   *out = nullptr;
   muonx_engine_config cfg = in;
+  if (cfg.slice_order != MUONX_SLICES_OLDEST_FIRST && cfg.slice_order != MUONX_SLICES_LEAST_WORK_FIRST) {
+    SetError("engine config: unknown slice_order");
+    return MUONX_ERR_INVALID;
+  }
   if (cfg.max_jobs == 0) cfg.max_jobs = 1024;
   if (cfg.shadow_per_vertex <= 0) cfg.shadow_per_vertex = 4;
   if (cfg.iterations_per_poll == 0) cfg.iterations_per_poll = 2;
@@ -219,13 +223,22 @@ int Scheduler::AdmitWork() {
         continue;
       }
     }
-    // otherwise the next slice of the oldest running job that has samples left
+    // otherwise the next slice of a running job that has samples left: the
+    // oldest one (jobs_ is in ticket order), or the one with the least work
+    // left
     SchedJob *next = nullptr;
+    uint64_t least = ~uint64_t(0);
     for (auto &kv : jobs_) {
       SchedJob &j = kv.second;
-      if (j.state == kActive && j.next_sample < uint32_t(j.spec.params.spp)) {
+      if (j.state != kActive || j.next_sample >= uint32_t(j.spec.params.spp)) continue;
+      if (cfg_.slice_order == MUONX_SLICES_OLDEST_FIRST) {
         next = &j;
         break;
+      }
+      const uint64_t left = uint64_t(uint32_t(j.spec.params.spp) - j.next_sample) * j.npix;
+      if (left < least) {
+        least = left;
+        next = &j;
       }
     }
     if (!next) break;

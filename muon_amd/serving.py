@@ -12,10 +12,14 @@ from . import _lib
 from ._lib import check, lib
 
 
+SLICES_OLDEST_FIRST = 0      # a running render's next slice: oldest render first (default)
+SLICES_LEAST_WORK_FIRST = 1  # ... or the render with the least work left (lower median, longer tail)
+
+
 def engine_config(path_slots=0, max_jobs=0, shadow_per_vertex=0, iterations_per_poll=0, film_budget_bytes=0,
-                  use_graph=True):
+                  use_graph=True, slice_order=SLICES_OLDEST_FIRST):
     return _lib.EngineConfig(path_slots, max_jobs, shadow_per_vertex, iterations_per_poll, film_budget_bytes,
-                             1 if use_graph else 0)
+                             1 if use_graph else 0, slice_order)
 
 
 @dataclasses.dataclass

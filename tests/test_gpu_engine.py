@@ -110,14 +110,16 @@ def test_keys_change_the_noise_not_the_estimate(ctx, resident):
     assert abs(d.mean()) < 6.0 * d.std() / np.sqrt(d.size)
 
 
-def test_server_results_equal_alone_under_load(ctx, resident):
+@pytest.mark.parametrize("order", [0, 1], ids=["oldest-first", "least-work-first"])
+def test_server_results_equal_alone_under_load(ctx, resident, order):
     """Requests arrive while others run; the film budget admits only a few
-    at a time, so later requests wait for films to free up."""
+    at a time, so later requests wait for films to free up. Either order of
+    continuing slices gives every request its image alone."""
     jobs = job_list(resident) * 3
     ref = [alone(ds, p) for ds, p, _ in jobs]
     biggest = max(p.width * p.height for _, p, _ in jobs) * 24
     cfg = engine_config(path_slots=8192, max_jobs=8, shadow_per_vertex=4, iterations_per_poll=2,
-                        film_budget_bytes=2 * biggest + 4096)
+                        film_budget_bytes=2 * biggest + 4096, slice_order=order)
     with Server(ctx, cfg) as srv:
         tickets = []
         for i, (ds, p, key) in enumerate(jobs):
@@ -175,6 +177,12 @@ def test_server_cancel(ctx, resident):
     assert np.array_equal(acc_c.cpu().numpy().view(np.uint32), ref_c.view(np.uint32))
     assert bool((sentinel == 7.0).all())
     assert st["cancelled"] == 1 and st["completed"] == 2 and st["film_bytes_in_use"] == 0
+
+
+def test_server_rejects_unknown_slice_order(ctx):
+    with pytest.raises(Exception, match="slice_order"):
+        with Server(ctx, engine_config(path_slots=4096, slice_order=7)):
+            pass
 
 
 def test_server_rejects_what_can_never_fit(ctx, resident):
