@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-sphere-field", action="store_true", help="render only the icosphere scene")
+    ap.add_argument("--only-sphere-field", action="store_true",
+                    help="of the renders, only the sphere field (for a profile of that workload)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving workloads")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -357,10 +359,12 @@ def main():
                 "mean_radiance": float(accum.mean().item()) / params.spp}
 
     render = None
-    if not args.no_render:
+    if not args.no_render and not args.only_sphere_field:
         log("bench: path-traced render 512x512 64spp depth 8")
         render = timed_render(ds, hs.params)
         log(f"bench: render {render}")
+    elif not args.no_render:
+        render = {}
     if not args.no_render and not args.no_sphere_field:
         log("bench: sphere field 1024x1024 32spp depth 8 (10,008 spheres, BRDF sampling, NEE off)")
         hs2 = HostScene.from_text(scenes.sphere_field_text())
@@ -368,7 +372,7 @@ def main():
         render["sphere_field"] = timed_render(ds2, hs2.params)
         log(f"bench: sphere field {render['sphere_field']}")
         del ds2
-    if not args.no_render:
+    if not args.no_render and not args.only_sphere_field:
         log("bench: Cornell box 256x256 16spp depth 5 (the reference's CPU case)")
         hs0 = HostScene.from_text(scenes.cornell_text(256, 256, 16, 5, seed=0))
         ds0 = DeviceScene(ctx, hs0)
