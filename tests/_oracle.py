@@ -74,12 +74,14 @@ class Scene:
         return float(out.value) if r == 1 else None
 
     def prim_edge(self, prim_id, o, d):
-        """min normalised barycentric of the ray's hit on the triangle (1 for spheres)."""
+        """Smallest normalised barycentric coordinate of the ray's hit on the
+        plane of triangle `prim_id` (negative outside the triangle, 1 for a
+        sphere); None for an unknown id or a ray parallel to the plane."""
         o = np.ascontiguousarray(o, np.float32)
         d = np.ascontiguousarray(d, np.float32)
         out = _c.c_float(0.0)
         r = self._lib.oracle_prim_edge(self.h, int(prim_id), _ptr(o), _ptr(d), _c.byref(out))
-        return float(out.value) if r == 0 else 0.0
+        return float(out.value) if r == 0 else None
 
     def camera(self):
         out = np.zeros(14, np.float32)

@@ -3,10 +3,15 @@ muon/acceleration.cc + muon/objects.cc), through the C ABI.
 
 Bar, exact traversal mode: primitive ids, distance bits and any-hit flags
 equal to the oracle's on every ray.
-Bar, fast traversal mode: primitive ids equal except (i) hits within 1e-6
-of a triangle edge and (ii) distance ties within 1e-5 relative — coincident
-surfaces (a face lying on another face) reach 2e-6 because each distance is
-computed through its own transform; where ids agree, distance bits agree."""
+Bar, fast traversal mode: primitive ids equal except (i) distance ties
+within 1e-5 relative — coincident surfaces (a face lying on another face)
+reach 2e-6 because each distance is computed through its own transform —
+and (ii) closer hits within 1e-6 (normalised barycentric) of a triangle
+edge, which the reference's rounded slab test can reject; the GPU's id must
+be a primitive the oracle's own test hits, and excused rays stay below 1e-4
+of the batch (tests/gpu_util.py). Where ids agree, distance bits agree.
+Any-hit: every ray occluded for the oracle is occluded on the GPU; extra
+occlusions (edge grazes) below 1e-4 of the batch."""
 import os
 
 import numpy as np
@@ -16,7 +21,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from muon_amd import Context, DeviceScene, HostScene, scenes  # noqa: E402
-from gpu_util import classify_mismatches, random_rays, scene_bounds  # noqa: E402
+from gpu_util import assert_anyhit_parity, assert_closest_parity, random_rays, scene_bounds  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -37,19 +42,19 @@ def _query(ds, o, d, tmax=None, exact=False):
     return prim.cpu().numpy(), t.cpu().numpy(), occ.cpu().numpy()
 
 
-def _check(orc, o, d, prim, t, tmax=None, occ=None):
+def _check(orc, o, d, prim, t, tmax=None, occ=None, max_tie_frac=1e-4):
     pr, tr, _, _, _ = orc.intersect(o, d)
-    bad = classify_mismatches(orc, o, d, prim, t, pr, tr, rel=1e-5)
-    assert not bad, f"{len(bad)} non-tie mismatches, first {bad[:5]}"
-    same = (prim == pr) & (pr >= 0)
-    rel = np.abs(t[same] - tr[same]) / np.maximum(np.abs(tr[same]), 1e-30)
-    assert rel.max(initial=0) <= 1e-5
-    exact = float(np.mean(t[same].view(np.uint32) == tr[same].view(np.uint32))) if same.any() else 1.0
+    _, exact, hits = assert_closest_parity(orc, o, d, prim, t, pr, tr, max_tie_frac=max_tie_frac)
     if occ is not None:
         oref, _ = orc.occluded(o, d, tmax)
-        assert np.array_equal(occ, oref)
-    return exact, int((pr >= 0).sum())
+        assert_anyhit_parity(occ, oref)
+    return exact, hits
 
+
+# Scenes with coincident faces, where two primitives tie at the same distance
+# and the visit order decides: the table legs of `transforms` end on the
+# table top's underside, the boxes of `cornell_normals` rest on its floor.
+COINCIDENT_TIE_FRAC = {"transforms": 2e-3, "cornell_normals": 2e-3}
 
 SCENES = ["intersection", "transforms", "spheres_raytrace", "cornell_raytrace", "cornell_normals", "ggx", "mis",
           "sphere"]
@@ -89,10 +94,10 @@ def test_reference_scene_queries(ctx, width, oracle, ref_dir, name):
     assert np.array_equal(occ, oref)
     # fast mode: equal outside ties / edge hits, distance bits equal where ids agree
     prim, t, _ = _query(ds, o, d)
-    exact_frac, _ = _check(orc, o, d, prim, t)
+    exact_frac, _ = _check(orc, o, d, prim, t, max_tie_frac=COINCIDENT_TIE_FRAC.get(name, 1e-4))
     assert exact_frac == 1.0
     _, _, occ = _query(ds, o, d, tmax)
-    assert (occ != oref).mean() < 1e-4
+    assert_anyhit_parity(occ, oref)
 
 
 def test_camera_rays_bit_exact(ctx, oracle, ref_dir):
@@ -149,20 +154,28 @@ def test_icosphere_microbench_parity(icosphere, width):
     assert exact > 0.9999
 
 
-def test_icosphere_full_microbench_invariants(icosphere, width):
-    """All 2^24 microbench rays: every hit is a valid primitive at positive
-    distance; the oracle re-test of a random subset agrees."""
+def test_icosphere_full_microbench_parity(icosphere):
+    """All 2^24 microbench rays (the benchmark's batch) with the default
+    launch configuration: closest-hit ids equal the oracle's on every ray
+    outside ties and edge grazes (excused below 1e-4 of the rays), distance
+    bits equal where ids agree; every hit a valid primitive at positive
+    finite distance."""
     hs, ds, orc = icosphere
     lo, hi = scenes.icosphere_world_bounds()
     o, d = scenes.microbench_rays(1 << 24, lo, hi, seed=0)
-    prim, t, _ = _query(ds, o, d)
+    dev = torch.device("cuda:0")
+    prim_t, t_t = ds.intersect_closest(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev))
+    torch.cuda.synchronize()
+    prim, t = prim_t.cpu().numpy(), t_t.cpu().numpy()
+    del prim_t, t_t
     hit = prim >= 0
     assert hit.mean() > 0.5
     assert (prim[hit] < hs.desc.num_prims).all() and (t[hit] > 0).all() and np.isfinite(t[hit]).all()
-    rng = np.random.default_rng(5)
-    idx = rng.choice(o.shape[0], 4096, replace=False)
-    pr, tr, _, _, _ = orc.intersect(o[idx], d[idx])
-    assert not classify_mismatches(orc, o[idx], d[idx], prim[idx], t[idx], pr, tr, rel=1e-5)
+    pr, tr, _, _, _ = orc.intersect(o, d, threads=16)
+    excused, exact, hits = assert_closest_parity(orc, o, d, prim, t, pr, tr)
+    print(f"2^24 rays: {hits} oracle hits, {int((prim != pr).sum())} id mismatches, {excused} excused, "
+          f"distance bits equal on {exact:.6f} of agreeing hits")
+    assert exact > 0.9999
 
 
 def _stacked_leaf_text(n_stack, n_rand=200, seed=3):

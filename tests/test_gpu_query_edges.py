@@ -11,7 +11,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from muon_amd import Context, DeviceScene, HostScene, scenes  # noqa: E402
-from gpu_util import classify_mismatches, random_rays, scene_bounds  # noqa: E402
+from gpu_util import assert_anyhit_parity, assert_closest_parity, random_rays, scene_bounds  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -54,13 +54,18 @@ def _run(ds, o, d, tmax=None, exact=False):
     return prim.cpu().numpy(), t.cpu().numpy(), occ.cpu().numpy()
 
 
-def _check_closest(orc, o, d, prim, t, pr, tr, exact):
+# The Cornell box's two boxes rest on its floor: their bottom faces and the
+# floor tie for rays that start inside a box and leave through its bottom.
+CORNELL_TIE_FRAC = 5e-3
+
+
+def _check_closest(orc, o, d, prim, t, pr, tr, exact, max_graze_frac=1e-4):
     if exact:
         assert np.array_equal(prim, pr)
         assert np.array_equal(t[pr >= 0].view(np.uint32), tr[pr >= 0].view(np.uint32))
     else:
-        bad = classify_mismatches(orc, o, d, prim, t, pr, tr, rel=1e-5)
-        assert not bad, bad[:5]
+        assert_closest_parity(orc, o, d, prim, t, pr, tr, max_tie_frac=CORNELL_TIE_FRAC,
+                              max_graze_frac=max_graze_frac)
 
 
 @pytest.mark.parametrize("n", [0, 1, 63, 65, 257, 4099])
@@ -77,7 +82,10 @@ def test_ragged_batch_sizes(cornell, width, n, exact):
     pr, tr, _, _, _ = orc.intersect(o, d)
     _check_closest(orc, o, d, prim, t, pr, tr, exact)
     oref, _ = orc.occluded(o, d, np.full(n, np.inf, np.float32))
-    assert np.array_equal(occ, oref) if exact else (occ != oref).sum() <= max(1, n // 1000)
+    if exact:
+        assert np.array_equal(occ, oref)
+    else:
+        assert_anyhit_parity(occ, oref)
 
 
 @pytest.mark.parametrize("exact", [True, False], ids=["exact", "fast"])
@@ -99,12 +107,18 @@ def test_axis_aligned_rays(cornell, width, exact):
     prim, t, occ = _run(ds, o, d, exact=exact)
     pr, tr, _, _, _ = orc.intersect(o, d)
     assert (pr >= 0).mean() > 0.5
-    _check_closest(orc, o, d, prim, t, pr, tr, exact)
+    # rays running inside wall planes from origins on them end on the walls'
+    # edges: grazes are the point of this test, not a rare event
+    _check_closest(orc, o, d, prim, t, pr, tr, exact, max_graze_frac=1e-2)
     oref, _ = orc.occluded(o, d, np.full(o.shape[0], np.inf, np.float32))
-    bad = np.nonzero(occ != oref)[0]
-    info = [(o[i].tolist(), d[i].tolist(), int(occ[i]), int(oref[i]), int(pr[i]), float(tr[i]), int(prim[i]))
-            for i in bad[:6]]
-    assert len(bad) == 0 if exact else len(bad) <= o.shape[0] // 1000, (len(bad), info)
+    if exact:
+        bad = np.nonzero(occ != oref)[0]
+        info = [(o[i].tolist(), d[i].tolist(), int(occ[i]), int(oref[i]), int(pr[i]), float(tr[i]), int(prim[i]))
+                for i in bad[:6]]
+        assert len(bad) == 0, (len(bad), info)
+    else:
+        # origins on the walls' planes graze boxes more often than random rays
+        assert_anyhit_parity(occ, oref, max_frac=1e-3)
 
 
 @pytest.mark.parametrize("exact", [True, False], ids=["exact", "fast"])
@@ -129,7 +143,10 @@ def test_tmax_limits(cornell, width, exact):
     assert (prim[tmax <= 0] == -1).all() and (occ[tmax <= 0] == 0).all()
     _check_closest(orc, o, d, prim, t, expect, tr, exact)
     oref, _ = orc.occluded(o, d, tmax)
-    assert np.array_equal(occ, oref) if exact else (occ != oref).mean() < 1e-3
+    if exact:
+        assert np.array_equal(occ, oref)
+    else:
+        assert_anyhit_parity(occ, oref, max_frac=1e-3)
 
 
 @pytest.mark.parametrize("accel,geom", [(0, "tris+sphere"), (1, "one sphere")], ids=["linear", "one-prim"])
