@@ -271,6 +271,19 @@ typedef struct muonx_query {
 /* General batched query: closest or any hit, either traversal mode. */
 int muonx_intersect(muonx_scene *scene, const muonx_query *q, void *stream);
 
+/* muonx_intersect that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Results
+ * are those of muonx_intersect. The traversal stacks' spill region and the
+ * ray-claim counter live in a 32 MiB static device array per GPU (held until
+ * the process ends) instead of context buffers; the grid is capped so that
+ * the spill fits, and a scene whose traversal stack needs more than the
+ * array holds for one block is rejected with MUONX_ERR_CAPACITY. Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads, other contexts) race and must be serialised by the
+ * caller. The caller passes the rays and outputs as for muonx_intersect. */
+int muonx_intersect_noalloc(muonx_scene *scene, const muonx_query *q, void *stream);
+
 /* ------------------------------------------------------------ render
  * Replaces the tile loop of muon::Renderer::Render (muon/renderer.cc:40-76):
  * Camera::CastRay + Integrator::Trace per sample + Film::SetPixel.

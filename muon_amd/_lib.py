@@ -120,6 +120,7 @@ SIGNATURES = [
     ("muonx_intersect_closest", C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
     ("muonx_intersect_any", C.c_int, [P, P, P, P, C.c_uint32, P, P]),
     ("muonx_intersect", C.c_int, [P, C.POINTER(Query), P]),
+    ("muonx_intersect_noalloc", C.c_int, [P, C.POINTER(Query), P]),
     ("muonx_render", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC), P]),
     ("muonx_render_host", C.c_int, [P, C.POINTER(RenderParams), P, C.POINTER(TraceStatsC)]),
     ("muonx_render_batch", C.c_int, [P, C.POINTER(Job), C.c_uint32, C.POINTER(EngineConfig),

@@ -9,6 +9,7 @@
 
 #include "../common/vecmath.h"
 #include "../host/host_scene.h"
+#include "query.h"
 #include "runtime.h"
 
 namespace muonx {
@@ -47,6 +48,8 @@ int muonx_ctx_create(int device, muonx_ctx **out) {
   MX_HIP(hipSetDevice(device));
   hipDeviceProp_t prop;
   MX_HIP(hipGetDeviceProperties(&prop, device));
+  const int rc = InitStaticScratch(device);
+  if (rc != MUONX_OK) return rc;
   auto *c = new (std::nothrow) muonx_ctx();
   if (!c) return MUONX_ERR_NOMEM;
   c->device = device;

@@ -12,7 +12,12 @@ constexpr int kQueryBlock = 128;
 
 int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, const float *ro, const float *rd,
                 const float *tmax, uint32_t n, int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters,
-                hipStream_t stream);
+                hipStream_t stream, bool noalloc = false);
+
+// Looks up the device address of the static scratch of
+// muonx_intersect_noalloc for `device` (which must be current); once per
+// device, at context creation.
+int InitStaticScratch(int device);
 
 }  // namespace muonx
 

@@ -6,6 +6,9 @@
 #include "traverse_launch.h"
 #include "wave.h"
 
+#include <mutex>
+#include <string>
+
 namespace muonx {
 
 template <int KIND, bool WW, int WIDTH, bool ANY, bool COUNT, bool EXACT>
@@ -146,25 +149,98 @@ struct QueryLaunch {
 };
 }  // namespace
 
+// Static scratch of muonx_intersect_noalloc: the claim counter of the
+// per-lane fetch kernels (the first 256 bytes) and the traversal stacks'
+// spill region (the rest). Every GPU holds its own copy until the process
+// ends; its device address is looked up once per device by
+// InitStaticScratch (with the device current), never used by name on the
+// host.
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+constexpr size_t kStaticScratchHead = 256;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+void *StaticScratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+}  // namespace
+
+int InitStaticScratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) {
+    SetError("static scratch: device index " + std::to_string(device) + " above the table of " +
+             std::to_string(kMaxScratchDevices));
+    return MUONX_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MUONX_OK;
+  void *p = nullptr;
+  MX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = p;
+  return MUONX_OK;
+}
+
+// Launches the query kernels. With `noalloc` the spill region and the claim
+// counter come from the static scratch (the grid is capped so that the spill
+// fits), the counter is cleared by a kernel rather than a memset node, and
+// no event or buffer is created.
 int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, const float *ro, const float *rd,
                 const float *tmax, uint32_t n, int32_t *prim, float *t, uint8_t *occ, unsigned long long *counters,
-                hipStream_t stream) {
+                hipStream_t stream, bool noalloc) {
   if (n == 0) return MUONX_OK;
   TraverseCfg cfg;
-  int rc = PlanTraverse(ctx, s, n, kQueryBlock, exact, cfg);
-  if (rc != MUONX_OK) return rc;
-  if ((rc = BeginSpill(ctx, cfg, stream)) != MUONX_OK) return rc;
+  unsigned char *sbase = nullptr;
+  int rc;
+  if (noalloc) {
+    sbase = static_cast<unsigned char *>(StaticScratch(ctx->device));
+    if (!sbase) {
+      SetError("muonx_intersect_noalloc: no static scratch address for device " + std::to_string(ctx->device));
+      return MUONX_ERR_STATE;
+    }
+    rc = PlanTraverseShape(ctx->tuning, ctx->num_cus, s.bvh_depth, s.wstack, n, kQueryBlock, cfg, exact);
+    if (rc != MUONX_OK) return rc;
+    if (cfg.gcap > 0) {
+      const size_t per_block = size_t(kQueryBlock) * size_t(cfg.gcap) * sizeof(uint2);
+      const size_t max_blocks = (kStaticScratchBytes - kStaticScratchHead) / per_block;
+      if (max_blocks == 0) {
+        SetError("muonx_intersect_noalloc: a traversal stack of " + std::to_string(cfg.kind + cfg.gcap) +
+                 " entries per lane needs " + std::to_string(per_block) + " bytes of spill per block, more than the " +
+                 std::to_string(kStaticScratchBytes - kStaticScratchHead) + "-byte static scratch");
+        return MUONX_ERR_CAPACITY;
+      }
+      cfg.grid = int(std::min<size_t>(size_t(cfg.grid), max_blocks));
+      cfg.threads = cfg.grid * kQueryBlock;
+      cfg.spill = reinterpret_cast<uint2 *>(sbase + kStaticScratchHead);
+    }
+  } else {
+    rc = PlanTraverse(ctx, s, n, kQueryBlock, exact, cfg);
+    if (rc != MUONX_OK) return rc;
+    if ((rc = BeginSpill(ctx, cfg, stream)) != MUONX_OK) return rc;
+  }
 #define MX_Q(A, C, E) DispatchTraverse(cfg, QueryLaunch<A, C, E>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, stream})
 #define MX_QE(A, C) DispatchExact(cfg, QueryLaunch<A, C, true>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, stream})
   const bool cnt = counters != nullptr;
   const int fetch = ctx->tuning.ray_fetch;
   if (!exact && fetch > 0 && n <= (1u << 31)) {  // the claim counter stays below 2^32 (n + chunk x waves)
     // per-lane ray fetch; the claim counter is ordered like the spill region
-    MX_HIP(ctx->fetch_ctr.Reserve(sizeof(uint32_t)));
-    if (!ctx->spill_done) MX_HIP(hipEventCreateWithFlags(&ctx->spill_done, hipEventDisableTiming));
-    MX_HIP(hipStreamWaitEvent(stream, ctx->spill_done, 0));
-    MX_HIP(hipMemsetAsync(ctx->fetch_ctr.ptr, 0, sizeof(uint32_t), stream));
-    uint32_t *next = ctx->fetch_ctr.as<uint32_t>();
+    uint32_t *next;
+    if (noalloc) {
+      next = reinterpret_cast<uint32_t *>(sbase);
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, next);
+    } else {
+      MX_HIP(ctx->fetch_ctr.Reserve(sizeof(uint32_t)));
+      if (!ctx->spill_done) MX_HIP(hipEventCreateWithFlags(&ctx->spill_done, hipEventDisableTiming));
+      MX_HIP(hipStreamWaitEvent(stream, ctx->spill_done, 0));
+      MX_HIP(hipMemsetAsync(ctx->fetch_ctr.ptr, 0, sizeof(uint32_t), stream));
+      next = ctx->fetch_ctr.as<uint32_t>();
+    }
 #define MX_QD(A, C) DispatchTraverse(cfg, QueryDynLaunch<A, C>{cfg, s, ro, rd, tmax, n, prim, t, occ, counters, next, fetch, stream})
     if (any) {
       if (cnt) MX_QD(true, true);
@@ -175,7 +251,7 @@ int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, con
     }
 #undef MX_QD
     MX_HIP(hipGetLastError());
-    MX_HIP(hipEventRecord(ctx->spill_done, stream));
+    if (!noalloc) MX_HIP(hipEventRecord(ctx->spill_done, stream));
     return MUONX_OK;
   }
   if (exact) {
@@ -193,30 +269,47 @@ int LaunchQuery(muonx_ctx *ctx, const DevSceneView &s, bool any, bool exact, con
 #undef MX_Q
 #undef MX_QE
   MX_HIP(hipGetLastError());
-  return EndSpill(ctx, cfg, stream);
+  return noalloc ? MUONX_OK : EndSpill(ctx, cfg, stream);
 }
 
 }  // namespace muonx
 
 using namespace muonx;
 
-extern "C" int muonx_intersect(muonx_scene *scene, const muonx_query *q, void *stream) {
+namespace {
+int CheckQuery(const char *fn, muonx_scene *scene, const muonx_query *q) {
   if (!scene || !q) {
-    SetError("muonx_intersect: null argument");
+    SetError(std::string(fn) + ": null argument");
     return MUONX_ERR_INVALID;
   }
   if (q->n && (!q->ray_o || !q->ray_d || (q->any_hit ? !q->occluded : !q->prim_id))) {
-    SetError("muonx_intersect: missing ray or output array");
+    SetError(std::string(fn) + ": missing ray or output array");
     return MUONX_ERR_INVALID;
   }
   if (q->mode != MUONX_TRAVERSAL_FAST && q->mode != MUONX_TRAVERSAL_EXACT) {
-    SetError("muonx_intersect: unknown traversal mode");
+    SetError(std::string(fn) + ": unknown traversal mode");
     return MUONX_ERR_INVALID;
   }
+  return MUONX_OK;
+}
+
+int RunQuery(muonx_scene *scene, const muonx_query *q, void *stream, bool noalloc) {
   std::lock_guard<std::mutex> lock(scene->ctx->mu);
   MX_HIP(hipSetDevice(scene->ctx->device));
-  return LaunchQuery(scene->ctx, scene->view, q->any_hit != 0, q->mode == MUONX_TRAVERSAL_EXACT, q->ray_o, q->ray_d, q->tmax,
-                     q->n, q->prim_id, q->t, q->occluded, q->counters, static_cast<hipStream_t>(stream));
+  return LaunchQuery(scene->ctx, scene->view, q->any_hit != 0, q->mode == MUONX_TRAVERSAL_EXACT, q->ray_o, q->ray_d,
+                     q->tmax, q->n, q->prim_id, q->t, q->occluded, q->counters, static_cast<hipStream_t>(stream),
+                     noalloc);
+}
+}  // namespace
+
+extern "C" int muonx_intersect(muonx_scene *scene, const muonx_query *q, void *stream) {
+  const int rc = CheckQuery("muonx_intersect", scene, q);
+  return rc != MUONX_OK ? rc : RunQuery(scene, q, stream, false);
+}
+
+extern "C" int muonx_intersect_noalloc(muonx_scene *scene, const muonx_query *q, void *stream) {
+  const int rc = CheckQuery("muonx_intersect_noalloc", scene, q);
+  return rc != MUONX_OK ? rc : RunQuery(scene, q, stream, true);
 }
 
 extern "C" int muonx_intersect_closest(muonx_scene *scene, const float *ro, const float *rd, const float *tmax,

@@ -52,12 +52,14 @@ __device__ __forceinline__ typename StackOf<KIND>::T make_stack(uint2 *lds, int 
   }
 }
 
-// Resolves the variant for a scene (binary depth, wide-stack bound) and
-// reserves the spill region in `spill`. `want_exact` selects the exact mode:
-// binary nodes and full LDS stacks (the deterministic modes are not
-// performance-critical and keep the simplest form).
-inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_depth, uint32_t wstack,
-                            uint64_t work_items, int block, DevBuf &spill, TraverseCfg &cfg, bool want_exact = false) {
+// Resolves the variant for a scene (binary depth, wide-stack bound): node
+// format, grid, stack kind and the spill entries per thread (cfg.gcap; the
+// spill region then needs cfg.threads * cfg.gcap uint2, not yet assigned).
+// `want_exact` selects the exact mode: binary nodes and full LDS stacks (the
+// deterministic modes are not performance-critical and keep the simplest
+// form). Allocates nothing.
+inline int PlanTraverseShape(const muonx_tuning &t, int num_cus, uint32_t bvh_depth, uint32_t wstack,
+                             uint64_t work_items, int block, TraverseCfg &cfg, bool want_exact = false) {
   const int cus = num_cus > 0 ? num_cus : 256;
   cfg.width = want_exact || t.bvh_width != 4 ? kNodesBinary : (t.node_bits == 8 ? kNodesWideQ : kNodesWide);
   const int bpc = std::max(1, t.blocks_per_cu);
@@ -76,8 +78,6 @@ inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_dep
   if (ring || (!want_exact && need > 96)) {
     cfg.kind = ring ? t.stack_entries : 16;
     cfg.gcap = need - cfg.kind;
-    MX_HIP(spill.Reserve(size_t(cfg.threads) * cfg.gcap * sizeof(uint2)));
-    cfg.spill = spill.as<uint2>();
     return MUONX_OK;
   }
   if (need <= 24) cfg.kind = 24;
@@ -87,6 +87,16 @@ inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_dep
     SetError("BVH depth " + std::to_string(need) + " exceeds the exact-mode traversal stack (96)");
     return MUONX_ERR_CAPACITY;
   }
+  return MUONX_OK;
+}
+
+// PlanTraverseShape, then reserves the spill region in `spill`.
+inline int PlanTraverseInto(const muonx_tuning &t, int num_cus, uint32_t bvh_depth, uint32_t wstack,
+                            uint64_t work_items, int block, DevBuf &spill, TraverseCfg &cfg, bool want_exact = false) {
+  const int rc = PlanTraverseShape(t, num_cus, bvh_depth, wstack, work_items, block, cfg, want_exact);
+  if (rc != MUONX_OK || cfg.gcap == 0) return rc;
+  MX_HIP(spill.Reserve(size_t(cfg.threads) * cfg.gcap * sizeof(uint2)));
+  cfg.spill = spill.as<uint2>();
   return MUONX_OK;
 }
 

@@ -133,7 +133,9 @@ class DeviceScene:
 
     # Device-pointer queries. o, d: torch float32 (n,3) on the GPU.
     def query(self, o, d, tmax=None, any_hit=False, exact=False, counters=None, stream=None, prim=None, t=None,
-              occ=None):
+              occ=None, noalloc=False):
+        """muonx_intersect (noalloc: muonx_intersect_noalloc, whose scratch
+        is a static device array, so the call allocates no device memory)."""
         import torch
         n = o.shape[0]
         if any_hit and occ is None:
@@ -147,7 +149,8 @@ class DeviceScene:
                        int(exact), None if prim is None else prim.data_ptr(), None if t is None else t.data_ptr(),
                        None if occ is None else occ.data_ptr(), None if counters is None else counters.data_ptr())
         s = torch.cuda.current_stream(o.device).cuda_stream if stream is None else stream
-        check(lib.muonx_intersect(self.h, C.byref(q), s))
+        fn = lib.muonx_intersect_noalloc if noalloc else lib.muonx_intersect
+        check(fn(self.h, C.byref(q), s))
         return occ if any_hit else (prim, t)
 
     def intersect_closest(self, o, d, tmax=None, prim=None, t=None, stream=None, exact=False):
