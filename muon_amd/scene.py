@@ -50,6 +50,14 @@ class HostScene:
     def warnings(self):
         return lib.muonx_host_scene_warnings(self.h)
 
+    @property
+    def output(self):
+        """The scene's output file name (muon/scene.h:36): the `output`
+        command's argument, or "raytrace.png"."""
+        out = C.c_char_p()
+        check(lib.muonx_host_scene_output(self.h, C.byref(out)))
+        return out.value.decode()
+
     def arrays(self):
         """Copies of the descriptor arrays (for tests and tools)."""
         d = self.desc
@@ -114,12 +122,26 @@ def autotune(dev_scene, job=None, rays=0):
 class DeviceScene:
     """A scene resident in HBM (muonx_scene)."""
 
-    def __init__(self, ctx, host_scene):
+    def __init__(self, ctx, host_scene, desc=None):
         out = C.c_void_p()
-        check(lib.muonx_scene_create(ctx.h, C.byref(host_scene.desc), C.byref(out)))
+        check(lib.muonx_scene_create(ctx.h, C.byref(desc if desc is not None else host_scene.desc), C.byref(out)))
         self.h = out
         self.ctx = ctx  # keeps the context alive
         self.params = host_scene.params
+
+    @classmethod
+    def from_arrays(cls, ctx, host_scene, **replace):
+        """A device scene from the host scene's descriptor with some arrays
+        replaced (numpy arrays of the descriptor's element type and length),
+        e.g. materials=... with other BRDF tags. The arrays are read during
+        the call only."""
+        desc = _lib.SceneDesc.from_buffer_copy(host_scene.desc)
+        keep = []
+        for name, arr in replace.items():
+            arr = np.ascontiguousarray(arr)
+            keep.append(arr)
+            setattr(desc, name, arr.ctypes.data)
+        return cls(ctx, host_scene, desc)
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:

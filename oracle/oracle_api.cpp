@@ -78,6 +78,17 @@ void *oracle_load_text(const char *text, int accel, int partition, long long def
   return h;
 }
 
+// oracle_load_text with `brdf lambertian` building the Lambertian BRDF
+// rather than the reference parser's Phong (a test hook for the Lambertian
+// restatement, muon/materials.cc:17-47).
+void *oracle_load_text_lambertian(const char *text, long long default_seed) {
+  auto h = new Handle();
+  ParseOptions o = MakeOpts(1, 2, default_seed);
+  o.true_lambertian = true;
+  h->scene = ParseSceneText(text, o);
+  return h;
+}
+
 void oracle_free(void *p) { delete static_cast<Handle *>(p); }
 
 // ints: W, H, spp, num_prims, integrator, nee, is, min_depth, max_depth, rr,

@@ -972,7 +972,8 @@ std::unique_ptr<Scene> ParseSceneText(const std::string &text, const ParseOption
         BrdfType t;
         // "lambertian" builds a Phong BRDF: the switch in muon/parser.cc:165-178
         // falls through from kLambertian into kPhong.
-        if (v == "lambertian" || v == "phong") t = BrdfType::kPhong;
+        if (v == "lambertian" && opts.true_lambertian) t = BrdfType::kLambertian;
+        else if (v == "lambertian" || v == "phong") t = BrdfType::kPhong;
         else if (v == "ggx") t = BrdfType::kGGX;
         else { bad(); break; }
         ws.GenMaterial();

@@ -260,6 +260,11 @@ struct ParseOptions {
   // When >= 0, replaces the std::random_device seed of the initial seed
   // generator (muon/parser.cc:181-182) so that runs are reproducible.
   int64_t default_seed = -1;
+  // Test hook, not a reference behaviour: `brdf lambertian` builds the
+  // Lambertian BRDF (muon/materials.cc:17-47) instead of the Phong BRDF the
+  // reference's parser falls through to, so that the Lambertian restatement
+  // can be checked.
+  bool true_lambertian = false;
 };
 
 std::unique_ptr<Scene> ParseSceneFile(const std::string &path, const ParseOptions &opts);

@@ -126,6 +126,8 @@ class Oracle:
         lib.oracle_load.argtypes = [_c.c_char_p, _c.c_int, _c.c_int, _c.c_longlong]
         lib.oracle_load_text.restype = _P
         lib.oracle_load_text.argtypes = [_c.c_char_p, _c.c_int, _c.c_int, _c.c_longlong]
+        lib.oracle_load_text_lambertian.restype = _P
+        lib.oracle_load_text_lambertian.argtypes = [_c.c_char_p, _c.c_longlong]
         lib.oracle_free.argtypes = [_P]
         lib.oracle_info.argtypes = [_P, _P, _P]
         lib.oracle_camera.argtypes = [_P, _P]
@@ -152,6 +154,11 @@ class Oracle:
 
     def load_text(self, text, accel=1, partition=2, seed=0):
         return Scene(self.lib, self.lib.oracle_load_text(text.encode(), accel, partition, seed))
+
+    def load_text_lambertian(self, text, seed=0):
+        """load_text with `brdf lambertian` building the Lambertian BRDF
+        (the reference's parser builds Phong for it)."""
+        return Scene(self.lib, self.lib.oracle_load_text_lambertian(text.encode(), seed))
 
     def num_tiles(self, w, h, spp, par):
         return int(self.lib.oracle_num_tiles(w, h, spp, par))

@@ -271,3 +271,32 @@ def test_classifier_excuses_edge_graze_and_tie_only(oracle):
 def test_allowed_excused_count():
     assert allowed(1e-4, 1 << 24) == 1677
     assert allowed(1e-4, 4099) == 1
+
+
+# ---------------------------------------------------------------- Lambertian BRDF
+def test_oracle_lambertian_equals_phong_without_specular(oracle):
+    """muon/materials.cc:17-47: the Lambertian BRDF is kd / pi with cosine
+    sampling, which ignores the specular colour; Phong with a zero specular
+    colour (reflectiveness 0, muon/materials.cc:113-123) is the same BRDF and
+    sampling, with one more random draw per sample. So the oracle's
+    Lambertian render with specular 0.5 and its Phong render with specular 0
+    estimate the same image: their means over 32 seeds agree within 5
+    standard errors, overall and per 2 x 2 block."""
+    import importlib.util
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(here, "golden", "make_lambertian_fixture.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    from muon_amd import scenes
+    n = 32
+    tiles = oracle.num_tiles(mk.W, mk.H, mk.SPP, mk.PAR)
+    skip = mk.H // tiles + (mk.H - (mk.H // tiles) * tiles)
+    lam = np.array([mk.block_means(oracle.load_text_lambertian(mk.scene_text(s)).render(mk.PAR)[0] / mk.SPP, skip)
+                    for s in range(n)])
+    phong = np.array([mk.block_means(oracle.load_text(scenes.cornell_text(mk.W, mk.H, mk.SPP, mk.DEPTH, seed=100 + s))
+                                     .render(mk.PAR)[0] / mk.SPP, skip) for s in range(n)])
+    z = (lam.mean(0) - phong.mean(0)) / np.sqrt(lam.var(0) / n + phong.var(0) / n)
+    assert np.abs(z).max() < 5.0, z
+    # and the same scene's Phong BRDF (specular 0.5 kept) is far brighter
+    assert oracle.load_text(mk.scene_text(0)).render(mk.PAR)[0].mean() > 2 * lam.mean() * mk.SPP
