@@ -178,6 +178,11 @@ int muonx_host_scene_desc(const muonx_host_scene *hs, muonx_scene_desc *out);
 int muonx_host_scene_params(const muonx_host_scene *hs, muonx_render_params *out);
 /* Count of malformed / unknown lines (muon logs them and continues). */
 int muonx_host_scene_warnings(const muonx_host_scene *hs);
+/* The scene's `output` file name (muon/scene.h:36, set by the `output`
+ * command, muon/parser.cc:309-318; default "raytrace.png"), which the render
+ * driver writes unless the caller overrides it (muon/renderer.cc:26-27).
+ * *out is valid until muonx_host_scene_destroy(). */
+int muonx_host_scene_output(const muonx_host_scene *hs, const char **out);
 
 /* ------------------------------------------------------------ device context / scene */
 /* One context per device; calls on a context are serialised internally. */

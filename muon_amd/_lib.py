@@ -109,6 +109,7 @@ SIGNATURES = [
     ("muonx_host_scene_desc", C.c_int, [P, C.POINTER(SceneDesc)]),
     ("muonx_host_scene_params", C.c_int, [P, C.POINTER(RenderParams)]),
     ("muonx_host_scene_warnings", C.c_int, [P]),
+    ("muonx_host_scene_output", C.c_int, [P, C.POINTER(C.c_char_p)]),
     ("muonx_ctx_create", C.c_int, [C.c_int, C.POINTER(P)]),
     ("muonx_ctx_destroy", None, [P]),
     ("muonx_ctx_set_tuning", C.c_int, [P, C.POINTER(Tuning)]),

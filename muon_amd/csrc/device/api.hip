@@ -162,6 +162,15 @@ int muonx_host_scene_params(const muonx_host_scene *hs, muonx_render_params *out
 
 int muonx_host_scene_warnings(const muonx_host_scene *hs) { return hs ? hs->sc.warnings : -1; }
 
+int muonx_host_scene_output(const muonx_host_scene *hs, const char **out) {
+  if (!hs || !out) {
+    SetError("muonx_host_scene_output: null argument");
+    return MUONX_ERR_INVALID;
+  }
+  *out = hs->sc.output.c_str();
+  return MUONX_OK;
+}
+
 // muon/film.cc:21-43
 int muonx_film_to_rgb8(const float *accum, int32_t w, int32_t h, int32_t spp, float gamma, uint8_t *rgb8) {
   if (!accum || !rgb8 || w <= 0 || h <= 0 || spp <= 0) {

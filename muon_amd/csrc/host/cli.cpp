@@ -1,7 +1,7 @@
 // muonx CLI: muon's flags (muon/muon.cc:12-21) over the GPU renderer.
 //   muonx --scene=path.muon [--output=out.png] [--acceleration=bvh|linear]
 //         [--partition_strategy=sah|midpoint|uniform] [--parallelism=N]
-//         [--stats=true|false] [--seed=N] [--device=N]
+//         [--stats=true|false] [--enable_feexcept[=true|false]] [--seed=N] [--device=N]
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -47,6 +47,7 @@ int main(int argc, char **argv) {
     } else if (Flag(argv[i], "parallelism", v)) opts.parallelism = uint32_t(std::strtoul(v.c_str(), nullptr, 10));
     else if (Flag(argv[i], "stats", v)) opts.show_stats = (v == "true" || v == "1");
     else if (Flag(argv[i], "nostats", v)) opts.show_stats = false;
+    else if (Flag(argv[i], "enable_feexcept", v)) opts.enable_feexcept = (v == "true" || v == "1");
     else if (Flag(argv[i], "seed", v)) opts.default_seed = std::strtoll(v.c_str(), nullptr, 10);
     else if (Flag(argv[i], "device", v)) opts.device = std::atoi(v.c_str());
     else {

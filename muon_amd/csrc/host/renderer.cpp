@@ -1,5 +1,6 @@
 #include "renderer.h"
 
+#include <cfenv>
 #include <chrono>
 #include <cstdio>
 #include <iomanip>
@@ -16,6 +17,24 @@ int Fail(const char *what) {
   return 1;
 }
 
+// Floating-point exception traps (FE_DIVBYZERO | FE_INVALID) switched on
+// for host-side stretches only, so that the GPU runtime's own host code
+// runs with the default environment.
+class HostFpTraps {
+ public:
+  explicit HostFpTraps(bool enabled) : enabled_(enabled) { On(); }
+  ~HostFpTraps() { Off(); }
+  void On() {
+    if (enabled_) feenableexcept(FE_DIVBYZERO | FE_INVALID);
+  }
+  void Off() {
+    if (enabled_) fedisableexcept(FE_DIVBYZERO | FE_INVALID);
+  }
+
+ private:
+  bool enabled_;
+};
+
 }  // namespace
 
 // muon/renderer.cc:18-85 with the tile loop replaced by muonx_render.
@@ -25,7 +44,13 @@ int Renderer::Render() const {
   muonx_options o{options_.acceleration, options_.partition_strategy, int32_t(options_.parallelism),
                   options_.show_stats ? 1 : 0, options_.default_seed};
   muonx_host_scene *hs = nullptr;
-  if (muonx_host_scene_parse_file(scene_file_.c_str(), &o, &hs) != MUONX_OK) return Fail("parse");
+  // muon/debug.cc:7-17: --enable_feexcept traps division by zero and
+  // invalid operations; here in the host-side parse, BVH build and film
+  // conversion (the render itself runs on the GPU)
+  HostFpTraps traps(options_.enable_feexcept);
+  const int prc = muonx_host_scene_parse_file(scene_file_.c_str(), &o, &hs);
+  traps.Off();
+  if (prc != MUONX_OK) return Fail("parse");
   muonx_scene_desc desc;
   muonx_render_params params;
   muonx_host_scene_desc(hs, &desc);
@@ -51,12 +76,15 @@ int Renderer::Render() const {
     result = Fail("render");
   } else {
     std::vector<uint8_t> rgb(accum.size());
+    traps.On();
     muonx_film_to_rgb8(accum.data(), params.width, params.height, params.spp, params.gamma, rgb.data());
+    traps.Off();
     // muon/renderer.cc:26-27: --output overrides the scene's output
     std::string out = options_.output;
     if (out.empty()) {
-      // the scene's own output name is not exposed through the params; fall back to muon's default
-      out = "raytrace.png";
+      const char *scene_out = nullptr;
+      muonx_host_scene_output(hs, &scene_out);
+      out = scene_out ? scene_out : "raytrace.png";
     }
     if (muonx_write_png(out.c_str(), rgb.data(), params.width, params.height) != MUONX_OK) result = Fail("write");
   }

@@ -17,6 +17,7 @@ struct Options {
   bool show_stats = true;
   int64_t default_seed = -1;
   int device = 0;
+  bool enable_feexcept = false;  // muon/debug.cc:7-17 (--enable_feexcept)
 };
 
 class Renderer {

@@ -125,9 +125,9 @@ class Renderer:
         accum, stats = render_device(ds, hs.params)
         film = Film(accum.cpu().numpy(), hs.params.spp, hs.params.gamma)
         stats.build_seconds = t1 - t0
-        output = o.output or None
-        if output:
-            film.write_png(output)
+        # muon/renderer.cc:26-27: --output overrides the scene's output
+        self.output = o.output or hs.output
+        film.write_png(self.output)
         if o.show_stats:
             print(stats, file=sys.stderr)
         self.film, self.stats = film, stats

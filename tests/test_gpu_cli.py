@@ -57,6 +57,38 @@ def test_cli_load_renders_like_inline_mesh(tmp_path):
     assert (a.sum(axis=2) > 0).mean() > 0.2
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("feexcept", [False, True], ids=["default", "enable_feexcept"])
+def test_cli_writes_the_scenes_output(tmp_path, ref_dir, feexcept):
+    """Without --output the render goes to the scene's `output` file
+    (muon/renderer.cc:26-27, relative to the working directory); with
+    --enable_feexcept (muon/debug.cc:7-17) the host-side work raises no
+    floating-point trap and the image is the same."""
+    text = open(os.path.join(ref_dir, "cornell_normals.muon")).read() + "\noutput from_scene.png\n"
+    (tmp_path / "scene.muon").write_text(text)
+    args = [CLI, "--scene=scene.muon"] + (["--enable_feexcept"] if feexcept else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert not (tmp_path / "raytrace.png").exists()
+    got, gold = _png(tmp_path / "from_scene.png"), _png(os.path.join(ref_dir, "cornell_normals.png"))
+    assert int((got != gold).any(axis=2).sum()) == 0
+
+
+@pytest.mark.gpu
+def test_python_renderer_writes_the_scenes_output(tmp_path, ref_dir, monkeypatch):
+    from muon_amd import Options, Renderer
+    text = open(os.path.join(ref_dir, "cornell_depth.muon")).read() + "\noutput depth_out.png\n"
+    (tmp_path / "scene.muon").write_text(text)
+    monkeypatch.chdir(tmp_path)
+    r = Renderer(str(tmp_path / "scene.muon"), Options(show_stats=False))
+    r.Render()
+    assert r.output == "depth_out.png"
+    got, gold = _png(tmp_path / "depth_out.png"), _png(os.path.join(ref_dir, "cornell_depth.png"))
+    assert int((got != gold).any(axis=2).sum()) == 0
+    Renderer(str(tmp_path / "scene.muon"), Options(output=str(tmp_path / "o.png"), show_stats=False)).Render()
+    assert np.array_equal(_png(tmp_path / "o.png"), got)
+
+
 def test_cli_flag_errors():
     r = _run([])
     assert r.returncode == 1 and "scene file is required" in r.stderr

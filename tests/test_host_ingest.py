@@ -133,3 +133,14 @@ def test_quirks(oracle):
     assert (h.params.width, h.params.height) == (320, 200)
     assert h.arrays()["materials"][0, 14] == 1  # Phong
     _compare(h, o)
+
+
+def test_scene_output_name():
+    """The `output` command sets the scene's output file (muon/parser.cc:309-318);
+    without it the default is raytrace.png (muon/defaults.h); a bare `output`
+    line is a malformed line (logged, output unchanged)."""
+    from muon_amd import HostScene
+    assert HostScene.from_text("film_size 4 4\n").output == "raytrace.png"
+    assert HostScene.from_text("output first.png\noutput second.png\n").output == "second.png"
+    hs = HostScene.from_text("output keep.png\noutput\n")
+    assert hs.output == "keep.png" and hs.warnings == 1
