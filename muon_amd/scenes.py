@@ -174,14 +174,15 @@ def _fmt_rows(arr):
     return [" ".join(F % x for x in row) for row in arr.astype(np.float64).tolist()]
 
 
-def icosphere_text(width=512, height=512, spp=64, max_depth=8, level=7, seed=0, radius=50.0):
+def icosphere_text(width=512, height=512, spp=64, max_depth=8, level=7, seed=0, radius=50.0, render_seed=None):
     """Displaced icosphere emitted at object radius 50 and placed with
     `translate 0 0.6 0; scale 0.008`: at world scale every triangle's
     2*area would be below the reference's 1e-4 parallel threshold
-    (muon/objects.cc:104-108) and the mesh would be invisible."""
+    (muon/objects.cc:104-108) and the mesh would be invisible. `seed` seeds
+    the displacement and, unless `render_seed` is given, the render."""
     verts, faces = displaced_icosphere(level, radius, 0.05, seed)
     lines = [f"# displaced icosphere, level {level}, {len(faces)} tris"]
-    cornell_header(lines, width, height, spp, max_depth, seed)
+    cornell_header(lines, width, height, spp, max_depth, seed if render_seed is None else render_seed)
     cornell_walls(lines)
     lines += ["push_transform", "translate 0 0.6 0", "scale 0.008 0.008 0.008", "brdf ggx", "roughness 0.3",
               "diffuse 0.1 0.1 0.1", "specular 0.6 0.6 0.6"]

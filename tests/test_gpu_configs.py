@@ -1,0 +1,111 @@
+"""The benchmark configurations as GPU tests.
+
+* single large mesh (level-7 displaced icosphere, 327,680 triangles, GGX
+  roughness 0.3, BRDF sampling + NEE + MIS, depth 8): the GPU path tracer's
+  mean radiance, overall and in 2 x 2 image blocks, over 64 independent
+  render keys equals the oracle's over 64 render seeds within 5 standard
+  errors (reduced film 32x32, 16 spp; tests/golden/make_c1_fixture.py wrote
+  the CPU side);
+* batched: the benchmark's 64-job mix (muon_amd.workloads.batched_jobs over
+  the resident pool scenes.POOL: resolution 128^2..1024^2, spp 4..64, depth
+  8) rendered as one batch equals the 64 plain renders bit for bit;
+* served: the benchmark's request mix (workloads.served_requests, 48
+  requests of 256^2..1024^2 at 8..128 spp) over the 8 resident serving scenes
+  (scenes.SERVE_POOL, about 5.7M primitives), submitted as a Poisson stream
+  while earlier requests render, equals each request rendered alone bit for
+  bit; every latency is positive and ordered submit <= admit <= done.
+"""
+import importlib.util
+import os
+import time
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from muon_amd import Context, DeviceScene, HostScene, scenes, workloads  # noqa: E402
+from muon_amd.renderer import render_device  # noqa: E402
+from muon_amd.serving import Server, engine_config, render_batch  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_spec = importlib.util.spec_from_file_location("make_c1_fixture", os.path.join(HERE, "golden", "make_c1_fixture.py"))
+c1 = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(c1)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return Context(0)
+
+
+def test_c1_icosphere_render_unbiased(ctx):
+    fx = np.load(os.path.join(HERE, "golden", "c1_icosphere32.npz"))
+    cpu = fx["block_means"]
+    n, skip = cpu.shape[0], int(fx["meta"][5])
+    hs = HostScene.from_text(c1.scene_text(0))
+    assert hs.desc.num_prims == 327680 + 12
+    ds = DeviceScene(ctx, hs)
+    accs, _, _ = render_batch(ctx, [(ds, hs.params, k) for k in range(n)])
+    gpu = np.array([c1.block_means(a.cpu().numpy() / c1.SPP, skip) for a in accs])
+    assert np.isfinite(gpu).all() and (gpu > 0).all()
+    se = np.sqrt(cpu.var(axis=0) / n + gpu.var(axis=0) / n)
+    z = (gpu.mean(axis=0) - cpu.mean(axis=0)) / se
+    assert np.abs(z).max() < 5.0, (z, gpu.mean(axis=0) / cpu.mean(axis=0))
+    g, c = gpu.mean(axis=(1, 2)), cpu.mean(axis=(1, 2))
+    zt = (g.mean() - c.mean()) / np.sqrt(g.var() / n + c.var() / n)
+    assert abs(zt) < 5.0, (zt, g.mean() / c.mean())
+
+
+def test_c3_batched_mix_equals_plain(ctx):
+    pool = workloads.build_pool(ctx, scenes.POOL)
+    jobs = workloads.batched_jobs(pool, 64, seed=7)
+    assert len({id(ds) for ds, _, _ in jobs}) >= 6
+    samples = sum(p.width * p.height * p.spp for _, p, _ in jobs)
+    film = sum(p.width * p.height * 24 + 256 for _, p, _ in jobs)
+    cfg = engine_config(max_jobs=128, film_budget_bytes=film + (1 << 20), iterations_per_poll=4)
+    accs, results, st = render_batch(ctx, jobs, cfg)
+    torch.cuda.synchronize()
+    assert all(r.status == 0 for r in results)
+    assert st.primary_rays >= samples  # every sample's camera ray (rays past the depth cut are not traced)
+    for i, (ds, p, _) in enumerate(jobs):
+        ref, _ = render_device(ds, p)
+        torch.cuda.synchronize()
+        assert torch.equal(accs[i], ref), f"job {i}: {p.width}x{p.height} x {p.spp} spp"
+        del ref
+
+
+def test_c4_served_mix_equals_alone(ctx):
+    pool = workloads.build_pool(ctx, scenes.SERVE_POOL)
+    assert sum(i["num_prims"] for _, _, _, _, i in pool) > 5_500_000
+    reqs = workloads.served_requests(pool, 48, seed=7)
+    gaps = workloads.poisson_gaps(len(reqs), rate=12.0, seed=7)
+    cfg = engine_config(max_jobs=256, film_budget_bytes=1 << 30, iterations_per_poll=2)
+    outs = []
+    with Server(ctx, cfg) as srv:
+        t_next = time.perf_counter()
+        tickets = []
+        for i, (ds, p) in enumerate(reqs):
+            t_next += gaps[i]
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(ds, p, i))
+        for t in tickets:
+            out = srv.wait(t, timeout_ms=120000)
+            assert out is not None, "request timed out"
+            outs.append(out)
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["film_bytes_in_use"] == 0
+    overlapped = 0
+    for i, ((ds, p), (acc, res)) in enumerate(zip(reqs, outs)):
+        assert res.status == 0 and 0 < res.submit_ns <= res.admit_ns <= res.done_ns
+        if i and outs[i - 1][1].done_ns > res.submit_ns:
+            overlapped += 1  # arrived while the previous request was still rendering
+        (ref,), _, _ = render_batch(ctx, [(ds, p, i)])  # alone: a batch of one with the request's key
+        torch.cuda.synchronize()
+        assert torch.equal(acc, ref), f"request {i}: {p.width}x{p.height} x {p.spp} spp"
+        del ref
+    assert overlapped >= 5
