@@ -25,7 +25,7 @@ namespace {
 __global__ void k_tune_rays(float *o, float *d, uint32_t n, float x0, float y0, float z0, float x1, float y1,
                             float z1, uint32_t seed) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t key = path_key(seed, 0x7475u, i, 0);
+    const PathKey key = path_key(seed, 0x7475u, i, 0);
     uint32_t ctr = 0;
     const float u0 = rng_uniform(key, ctr), u1 = rng_uniform(key, ctr), u2 = rng_uniform(key, ctr);
     const float u3 = rng_uniform(key, ctr), u4 = rng_uniform(key, ctr);

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "engine.h"
+#include "engine_plan.h"
 #include "hitrec.h"
 #include "wave.h"
 #include "shading.h"
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
           ps.rad[3 * i + 2] = rad.z;
           if (job.nee != MUONX_NEE_OFF) {
             // ShadeDirectNEE (muon/integration.cc:340-466): one shadow ray per light sample
-            const uint32_t key = path_key(job.seed, job.key, ps.pixel[i], ps.sample[i]);
+            const PathKey key = path_key(job.seed, job.key, ps.pixel[i], ps.sample[i]);
             uint32_t ctr = ps.ctr[i];
             const bool mis = job.nee == MUONX_NEE_MIS;
             int k = 0;
@@ -191,9 +192,10 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
     }
     // the shadow rays to trace, compacted (zero contributions are skipped)
     uint32_t si = block_append_n<kShadeWaves>(nshadow, &a.c->shadow_count.v, wsum);
-    for (uint64_t m = vmask; m; m &= m - 1) a.s_queue[si++] = uint32_t(size_t(__ffsll((unsigned long long)m) - 1) * P + i);
+    for (uint64_t m = vmask; m; m &= m - 1)
+      a.s_queue[si++] = ShadowSlot(uint32_t(__ffsll((unsigned long long)m) - 1), a.capacity, i);
     for (int k = 64; nshadow && k < K; ++k) {
-      const uint32_t sl = uint32_t(k * P + i);
+      const uint32_t sl = ShadowSlot(uint32_t(k), a.capacity, i);
       if (a.s_valid[sl]) a.s_queue[si++] = sl;
     }
   }
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
       const V3 rd{qd.x, qd.y, qd.z};
       pixel = ps.pixel[i];
       sample = ps.sample[i];
-      const uint32_t key = path_key(job.seed, job.key, pixel, sample);
+      const PathKey key = path_key(job.seed, job.key, pixel, sample);
       ctr = ps.ctr[i];
       T = V3{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
       if (depth >= job.min_depth && job.nee == MUONX_NEE_MIS) {
@@ -601,9 +603,7 @@ __global__ void k_eng_plan(EngArgs a, int cur) {
     c->rays[id] += sum;
   }
   const uint32_t next = c->qcount[cur ^ 1].v;
-  const unsigned long long avail = c->total_items > c->cursor ? c->total_items - c->cursor : 0ull;
-  const uint32_t room = a.capacity - next;
-  const uint32_t nfill = uint32_t(avail < room ? avail : room);
+  const uint32_t nfill = FillCount(c->total_items, c->cursor, a.capacity, next);
   c->plan_n = nfill;
   c->plan_base = next;
   c->plan_cursor = c->cursor;
@@ -622,17 +622,10 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
   for (uint32_t j = blockIdx.x * kEngBlock + threadIdx.x; j < n; j += gridDim.x * kEngBlock) {
     const unsigned long long item = cursor + j;
     // last admitted job whose first item is <= item
-    unsigned long long l = lo, h = hi;  // invariant: job_first[l] <= item < job_first[h]
-    while (h - l > 1) {
-      const unsigned long long m = l + (h - l) / 2;
-      if (a.job_first[m % a.max_jobs] <= item) l = m;
-      else h = m;
-    }
-    const uint32_t js = uint32_t(l % a.max_jobs);
+    const uint32_t js = uint32_t(FindJobSeq(a.job_first, a.max_jobs, lo, hi, item) % a.max_jobs);
     const JobDev &job = a.jobs[js];
-    const unsigned long long local = item - job.first_item;
-    const uint32_t sample = job.sample_base + uint32_t(local / job.npix);
-    const uint32_t pixel = uint32_t(local % job.npix);
+    const ItemPos ip = ItemPosition(item - job.first_item, job.npix, job.sample_base);
+    const uint32_t sample = ip.sample, pixel = ip.pixel;
     const int px = int(pixel % uint32_t(job.cam.width)), py = int(pixel / uint32_t(job.cam.width));
     float x, y;
     sample_position(px, py, int(sample), path_key(job.seed, job.key, pixel, sample), x, y);
@@ -672,37 +665,63 @@ T *Carve(uintptr_t &p, size_t count) {
   return r;
 }
 
-size_t CarvePaths(uintptr_t base, size_t P, EngArgs &a) {
-  uintptr_t p = base;
-  for (auto &st : a.st) {
-    st.job = Carve<uint32_t>(p, P);
-    st.pixel = Carve<uint32_t>(p, P);
-    st.sample = Carve<uint32_t>(p, P);
-    st.ctr = Carve<uint32_t>(p, P);
-    st.depth = Carve<int32_t>(p, P);
-    st.rad = Carve<float>(p, 3 * P);
-    st.thr = Carve<float>(p, 3 * P);
+// Assigns the engine's arrays from `base` in the order and sizes of
+// kPathArrays / kShadowArrays (engine_plan.h); returns the buffer's bytes.
+// A null base only measures. Returns 0 if an array's element type does not
+// have the size the layout gives it.
+class Carver {
+ public:
+  Carver(uintptr_t base, const ArraySpec *spec, int count, size_t rows) : base_(base), spec_(spec), count_(count) {
+    bytes_ = LayoutOffsets(spec, count, rows, off_);
   }
-  a.q_ray[0] = Carve<float4>(p, 2 * P);
-  a.q_ray[1] = Carve<float4>(p, 2 * P);
-  a.m_ray = Carve<float4>(p, 2 * P);
-  a.hit_prim = Carve<int32_t>(p, P);
-  a.nidx = Carve<uint32_t>(p, P);
-  a.m_fac = Carve<float>(p, 3 * P);
-  a.m_hit = Carve<int32_t>(p, P);
-  a.shift = Carve<float4>(p, P);
-  a.hnorm = Carve<float4>(p, P);
-  return size_t(p - base);
+  template <typename T>
+  void operator()(T *&ptr) {
+    if (k_ >= count_ || spec_[k_].elem_bytes != sizeof(T)) ok_ = false;
+    ptr = reinterpret_cast<T *>(base_ + (k_ < count_ ? off_[k_] : 0));
+    ++k_;
+  }
+  size_t bytes() const { return ok_ && k_ == count_ ? bytes_ : 0; }
+
+ private:
+  uintptr_t base_;
+  const ArraySpec *spec_;
+  int count_, k_ = 0;
+  bool ok_ = true;
+  size_t off_[kNumPathArrays > kNumShadowArrays ? kNumPathArrays : kNumShadowArrays] = {};
+  size_t bytes_ = 0;
+};
+
+size_t CarvePaths(uintptr_t base, size_t P, EngArgs &a) {
+  Carver c(base, kPathArrays, kNumPathArrays, P);
+  for (auto &st : a.st) {
+    c(st.job);
+    c(st.pixel);
+    c(st.sample);
+    c(st.ctr);
+    c(st.depth);
+    c(st.rad);
+    c(st.thr);
+  }
+  c(a.q_ray[0]);
+  c(a.q_ray[1]);
+  c(a.m_ray);
+  c(a.hit_prim);
+  c(a.nidx);
+  c(a.m_fac);
+  c(a.m_hit);
+  c(a.shift);
+  c(a.hnorm);
+  return c.bytes();
 }
 
 size_t CarveShadow(uintptr_t base, size_t P, size_t K, EngArgs &a) {
-  uintptr_t p = base;
-  a.s_dir = Carve<float4>(p, P * K);
-  a.s_contrib = Carve<float>(p, 3 * P * K);
-  a.s_vis = Carve<uint8_t>(p, P * K);
-  a.s_valid = Carve<uint8_t>(p, P * K);
-  a.s_queue = Carve<uint32_t>(p, P * K);
-  return size_t(p - base);
+  Carver c(base, kShadowArrays, kNumShadowArrays, P * K);
+  c(a.s_dir);
+  c(a.s_contrib);
+  c(a.s_vis);
+  c(a.s_valid);
+  c(a.s_queue);
+  return c.bytes();
 }
 
 struct IterLaunch {
@@ -744,7 +763,7 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
     SetError("engine: capacity, max_jobs and kmax must be positive");
     return MUONX_ERR_INVALID;
   }
-  if (capacity > (1u << 30) || uint64_t(capacity) * uint64_t(kmax) > (uint64_t(1) << 31)) {
+  if (capacity > (1u << 30) || uint64_t(capacity) * uint64_t(kmax) > kMaxShadowSlots) {
     SetError("engine: paths in flight x shadow rays per vertex must stay below 2^31");
     return MUONX_ERR_INVALID;
   }
@@ -772,6 +791,10 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
     const uintptr_t b1 = pass ? reinterpret_cast<uintptr_t>(e->shadow_.ptr) : 0;
     path_bytes = CarvePaths(b0, capacity, a);
     shadow_bytes = CarveShadow(b1, capacity, size_t(kmax), a);
+    if (path_bytes == 0 || shadow_bytes == 0) {
+      SetError("engine: buffer layout does not match the path arrays");
+      return fail(MUONX_ERR_STATE);
+    }
     if (pass == 0 &&
         (e->paths_.Alloc(path_bytes) != hipSuccess || e->shadow_.Alloc(shadow_bytes) != hipSuccess ||
          e->counters_.Alloc(sizeof(EngCounters)) != hipSuccess ||

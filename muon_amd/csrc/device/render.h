@@ -43,7 +43,7 @@ __device__ __forceinline__ void cast_ray(const CamParams &c, float x, float y, V
 // jittered uniformly inside the pixel (counter-based draws instead of the
 // per-tile mt19937 stream), with the same guard against rounding onto the
 // next pixel.
-__device__ __forceinline__ void sample_position(int px, int py, int sidx, uint32_t key, float &x, float &y) {
+__device__ __forceinline__ void sample_position(int px, int py, int sidx, PathKey key, float &x, float &y) {
   if (sidx == 0) {
     x = float(px) + 0.5f;
     y = float(py) + 0.5f;

@@ -1,0 +1,111 @@
+// Host-side policies of the scheduler that do not touch the device: the
+// first-fit range allocator behind the film pool (admission by HBM
+// footprint) and the choice of the next slice of samples to hand to the
+// engine. Plain C++, checked on the CPU.
+#ifndef MUONX_DEVICE_SCHED_POLICY_H_
+#define MUONX_DEVICE_SCHED_POLICY_H_
+
+#include <cstdint>
+#include <iterator>
+#include <map>
+#include <vector>
+
+namespace muonx {
+
+// First-fit allocator of aligned byte ranges in [0, capacity), with free
+// ranges coalesced on release.
+class RangeAllocator {
+ public:
+  static constexpr uint64_t kAlign = 256;
+  static uint64_t Round(uint64_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+  void Reset(uint64_t capacity) {
+    cap_ = Round(capacity);
+    used_ = 0;
+    free_.clear();
+    if (cap_) free_[0] = cap_;
+  }
+  uint64_t capacity() const { return cap_; }
+  uint64_t in_use() const { return used_; }
+  // Returns false when no free range is large enough.
+  bool Alloc(uint64_t bytes, uint64_t *offset) {
+    bytes = Round(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      *offset = it->first;
+      const uint64_t rest = it->second - bytes, at = it->first + bytes;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      used_ += bytes;
+      return true;
+    }
+    return false;
+  }
+  void Free(uint64_t offset, uint64_t bytes) {
+    bytes = Round(bytes);
+    used_ -= bytes;
+    auto it = free_.emplace(offset, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {  // merge with the following range
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {  // merge with the preceding range
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  const std::map<uint64_t, uint64_t> &free_ranges() const { return free_; }
+
+ private:
+  uint64_t cap_ = 0, used_ = 0;
+  std::map<uint64_t, uint64_t> free_;  // offset -> bytes
+};
+
+// A running render as the slice choice sees it (in arrival order).
+struct SliceCandidate {
+  uint64_t work_left;        // pixels x samples not yet handed to the engine (0: nothing left)
+  uint64_t last_slice_iter;  // engine iteration at which it last got a slice
+};
+
+// A running render that got no slice for this many engine iterations goes
+// ahead of new admissions and of the slice order, so that a steady stream
+// of new or small requests cannot starve it.
+constexpr uint64_t kSliceAgeIterations = 64;
+
+enum class SliceChoice { kAdmitPending, kRunning, kNone };
+
+// Chooses what the next slice of the engine's backlog is for: the oldest
+// running render that has waited kSliceAgeIterations or more (`*index`);
+// else a pending render whose film fits (its first slice); else a running
+// render with work left, the oldest (least_work_first false) or the one
+// with the least work left.
+inline SliceChoice PickSlice(const std::vector<SliceCandidate> &running, bool pending_fits, bool least_work_first,
+                             uint64_t now_iter, size_t *index) {
+  for (size_t k = 0; k < running.size(); ++k) {
+    if (running[k].work_left && now_iter - running[k].last_slice_iter >= kSliceAgeIterations) {
+      *index = k;
+      return SliceChoice::kRunning;
+    }
+  }
+  if (pending_fits) return SliceChoice::kAdmitPending;
+  bool found = false;
+  for (size_t k = 0; k < running.size(); ++k) {
+    if (!running[k].work_left) continue;
+    if (!least_work_first) {
+      *index = k;
+      return SliceChoice::kRunning;
+    }
+    if (!found || running[k].work_left < running[*index].work_left) {
+      *index = k;
+      found = true;
+    }
+  }
+  return found ? SliceChoice::kRunning : SliceChoice::kNone;
+}
+
+}  // namespace muonx
+
+#endif

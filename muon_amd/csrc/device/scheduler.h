@@ -12,29 +12,30 @@
 #include <vector>
 
 #include "engine.h"
+#include "sched_policy.h"
 
 namespace muonx {
 
-// First-fit range allocator over one device buffer (films of in-flight jobs).
+// Films of in-flight jobs: one device buffer carved by a first-fit range
+// allocator (sched_policy.h).
 class FilmPool {
  public:
-  static constexpr uint64_t kAlign = 256;
+  static constexpr uint64_t kAlign = RangeAllocator::kAlign;
   int Init(uint64_t bytes);
-  uint64_t capacity() const { return cap_; }
-  uint64_t in_use() const { return used_; }
-  bool empty() const { return used_ == 0; }
+  uint64_t capacity() const { return ranges_.capacity(); }
+  uint64_t in_use() const { return ranges_.in_use(); }
+  bool empty() const { return ranges_.in_use() == 0; }
   // Returns false when no free range is large enough.
-  bool Alloc(uint64_t bytes, uint64_t *offset);
-  void Free(uint64_t offset, uint64_t bytes);
+  bool Alloc(uint64_t bytes, uint64_t *offset) { return ranges_.Alloc(bytes, offset); }
+  void Free(uint64_t offset, uint64_t bytes) { ranges_.Free(offset, bytes); }
   unsigned long long *at(uint64_t offset) const {
     return reinterpret_cast<unsigned long long *>(buf_.as<char>() + offset);
   }
-  static uint64_t Round(uint64_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+  static uint64_t Round(uint64_t b) { return RangeAllocator::Round(b); }
 
  private:
   DevBuf buf_;
-  uint64_t cap_ = 0, used_ = 0;
-  std::map<uint64_t, uint64_t> free_;  // offset -> bytes, coalesced
+  RangeAllocator ranges_;
 };
 
 struct SchedJob {
@@ -45,6 +46,7 @@ struct SchedJob {
   uint64_t film_off = 0, film_bytes = 0;
   uint32_t next_sample = 0;  // samples handed to the engine so far
   uint32_t open_slices = 0;  // of those, slices not finished yet
+  uint64_t last_slice_iter = 0;  // engine iteration of its latest slice
   int state = 0;  // kPending, kActive, kResolving, kDone
   muonx_job_result result{};
   hipEvent_t ev = nullptr;  // resolve finished
@@ -84,7 +86,9 @@ class Scheduler {
   // slice comes before further slices of running jobs (which continue
   // oldest first), so a small request waits behind about one queue fill,
   // not behind the whole of a large one, while large requests still finish
-  // in arrival order. Images do not depend on the slicing.
+  // in arrival order. A running job that got no slice for
+  // kSliceAgeIterations goes first (PickSlice), so no stream of new or
+  // smaller requests starves it. Images do not depend on the slicing.
   int Step(std::vector<SchedJob> *finished);
   bool Idle() const { return pending_.empty() && jobs_.empty(); }
   size_t pending() const { return pending_.size(); }

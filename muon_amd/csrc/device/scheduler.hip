@@ -1,5 +1,6 @@
 // Scheduler over the persistent engine (see scheduler.h).
 #include "scheduler.h"
+#include "engine_plan.h"
 
 #include <algorithm>
 #include <chrono>
@@ -17,43 +18,8 @@ uint64_t NowNs() {
 int FilmPool::Init(uint64_t bytes) {
   bytes = Round(std::max<uint64_t>(bytes, kAlign));
   MX_HIP(buf_.Alloc(bytes));
-  cap_ = bytes;
-  used_ = 0;
-  free_.clear();
-  free_[0] = bytes;
+  ranges_.Reset(bytes);
   return MUONX_OK;
-}
-
-bool FilmPool::Alloc(uint64_t bytes, uint64_t *offset) {
-  bytes = Round(bytes);
-  for (auto it = free_.begin(); it != free_.end(); ++it) {
-    if (it->second < bytes) continue;
-    *offset = it->first;
-    const uint64_t rest = it->second - bytes, at = it->first + bytes;
-    free_.erase(it);
-    if (rest) free_[at] = rest;
-    used_ += bytes;
-    return true;
-  }
-  return false;
-}
-
-void FilmPool::Free(uint64_t offset, uint64_t bytes) {
-  bytes = Round(bytes);
-  used_ -= bytes;
-  auto it = free_.emplace(offset, bytes).first;
-  auto next = std::next(it);
-  if (next != free_.end() && it->first + it->second == next->first) {  // merge with the following range
-    it->second += next->second;
-    free_.erase(next);
-  }
-  if (it != free_.begin()) {  // merge with the preceding range
-    auto prev = std::prev(it);
-    if (prev->first + prev->second == it->first) {
-      prev->second += it->second;
-      free_.erase(it);
-    }
-  }
 }
 
 // ---------------------------------------------------------------- scheduler
@@ -62,10 +28,6 @@ namespace {
 uint64_t FilmBytes(const muonx_render_params &p) {
   return FilmPool::Round(uint64_t(p.width) * uint64_t(p.height) * 3 * sizeof(unsigned long long));
 }
-
-// Device bytes of one path in flight (two queue-ordered states 88, rays and
-// per-path terms 152, 34 per shadow ray), as Engine::Create lays them out.
-uint64_t PathSlotBytes(int kmax) { return 88 + 152 + 34 * uint64_t(kmax); }
 
 }  // namespace
 
@@ -195,54 +157,52 @@ int Scheduler::AdmitSlice(SchedJob &j) {
   open_items_ += sl.nitems;
   j.next_sample += n;
   ++j.open_slices;
+  j.last_slice_iter = iterations_;
   return MUONX_OK;
 }
 
 int Scheduler::AdmitWork() {
   const uint64_t backlog = 3 * uint64_t(cfg_.path_slots) / 2;
+  std::vector<SliceCandidate> cands;
+  std::vector<SchedJob *> running;
   while (open_items_ < backlog && eng_->seq_hi() - eng_->seq_lo() < eng_->max_jobs()) {
-    // the oldest waiting job, once its film fits (admission by film memory,
-    // in arrival order)
-    if (!pending_.empty()) {
-      SchedJob &p = pending_.front();
-      uint64_t off = 0;
-      if (films_.Alloc(p.film_bytes, &off)) {
-        uint32_t slot = 0;
-        int rc = eng_->AddScene(p.spec.scene, &slot);
-        if (rc != MUONX_OK) {
-          films_.Free(off, p.film_bytes);
-          return rc;
-        }
-        p.film_off = off;
-        p.scene_slot = slot;
-        p.state = kActive;
-        p.result.admit_ns = NowNs();
-        SchedJob &j = jobs_.emplace(p.ticket, p).first->second;
-        pending_.pop_front();
-        if ((rc = AdmitSlice(j)) != MUONX_OK) return rc;
-        continue;
-      }
-    }
-    // otherwise the next slice of a running job that has samples left: the
-    // oldest one (jobs_ is in ticket order), or the one with the least work
-    // left
-    SchedJob *next = nullptr;
-    uint64_t least = ~uint64_t(0);
-    for (auto &kv : jobs_) {
+    cands.clear();
+    running.clear();
+    for (auto &kv : jobs_) {  // ticket (arrival) order
       SchedJob &j = kv.second;
-      if (j.state != kActive || j.next_sample >= uint32_t(j.spec.params.spp)) continue;
-      if (cfg_.slice_order == MUONX_SLICES_OLDEST_FIRST) {
-        next = &j;
-        break;
-      }
-      const uint64_t left = uint64_t(uint32_t(j.spec.params.spp) - j.next_sample) * j.npix;
-      if (left < least) {
-        least = left;
-        next = &j;
-      }
+      if (j.state != kActive) continue;
+      const uint32_t spp = uint32_t(j.spec.params.spp);
+      const uint64_t left = j.next_sample < spp ? uint64_t(spp - j.next_sample) * j.npix : 0;
+      cands.push_back(SliceCandidate{left, j.last_slice_iter});
+      running.push_back(&j);
     }
-    if (!next) break;
-    const int rc = AdmitSlice(*next);
+    // the oldest waiting job's film, if it fits (admission by film memory,
+    // in arrival order)
+    uint64_t off = 0;
+    const bool fits = !pending_.empty() && films_.Alloc(pending_.front().film_bytes, &off);
+    size_t idx = 0;
+    const SliceChoice ch =
+        PickSlice(cands, fits, cfg_.slice_order == MUONX_SLICES_LEAST_WORK_FIRST, iterations_, &idx);
+    if (ch != SliceChoice::kAdmitPending && fits) films_.Free(off, pending_.front().film_bytes);
+    if (ch == SliceChoice::kAdmitPending) {
+      SchedJob &p = pending_.front();
+      uint32_t slot = 0;
+      int rc = eng_->AddScene(p.spec.scene, &slot);
+      if (rc != MUONX_OK) {
+        films_.Free(off, p.film_bytes);
+        return rc;
+      }
+      p.film_off = off;
+      p.scene_slot = slot;
+      p.state = kActive;
+      p.result.admit_ns = NowNs();
+      SchedJob &j = jobs_.emplace(p.ticket, p).first->second;
+      pending_.pop_front();
+      if ((rc = AdmitSlice(j)) != MUONX_OK) return rc;
+      continue;
+    }
+    if (ch == SliceChoice::kNone) break;
+    const int rc = AdmitSlice(*running[idx]);
     if (rc != MUONX_OK) return rc;
   }
   return MUONX_OK;

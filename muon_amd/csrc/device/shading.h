@@ -34,14 +34,14 @@ __device__ __forceinline__ V3 spherical(float ct, float st, float phi) {
 }
 
 // muon/hemisphere_sampling.cc:8-22 (theta = acos(r1))
-__device__ __forceinline__ V3 sample_hemisphere(V3 n, uint32_t key, uint32_t &ctr) {
+__device__ __forceinline__ V3 sample_hemisphere(V3 n, PathKey key, uint32_t &ctr) {
   const float r1 = rng_uniform(key, ctr);
   const float r2 = rng_uniform(key, ctr);
   return rotate_to_frame(spherical(r1, sqrtf(gmax(1.0f - r1 * r1, 0.0f)), 2.0f * kPi * r2), n);
 }
 
 // muon/hemisphere_sampling.cc:24-38 (theta = acos(sqrt(r1)))
-__device__ __forceinline__ V3 sample_cosine(V3 n, uint32_t key, uint32_t &ctr) {
+__device__ __forceinline__ V3 sample_cosine(V3 n, PathKey key, uint32_t &ctr) {
   const float r1 = rng_uniform(key, ctr);
   const float r2 = rng_uniform(key, ctr);
   return rotate_to_frame(spherical(sqrtf(r1), sqrtf(1.0f - r1), 2.0f * kPi * r2), n);
@@ -86,7 +86,7 @@ __device__ __forceinline__ float smith_g1(V3 v, V3 n, float rough) {
 }
 
 // BRDF::Sample: Lambertian muon/materials.cc:17-24, Phong :49-71, GGX :180-205.
-__device__ __forceinline__ V3 brdf_sample(const Mat &m, V3 rd, V3 n, uint32_t key, uint32_t &ctr) {
+__device__ __forceinline__ V3 brdf_sample(const Mat &m, V3 rd, V3 n, PathKey key, uint32_t &ctr) {
   if (m.brdf == MUONX_BRDF_LAMBERTIAN) return sample_cosine(n, key, ctr);
   const float r0 = rng_uniform(key, ctr);
   if (r0 > m.refl) return sample_cosine(n, key, ctr);
@@ -205,7 +205,7 @@ __device__ __forceinline__ float is_pdf(int mode, const Mat &m, V3 wi, V3 rd, V3
 }
 
 // PathTracer::SampleReflection (muon/integration.cc:503-562).
-__device__ __forceinline__ bool sample_reflection(int mode, const Mat &m, V3 rd, V3 n, V3 T, uint32_t key,
+__device__ __forceinline__ bool sample_reflection(int mode, const Mat &m, V3 rd, V3 n, V3 T, PathKey key,
                                                   uint32_t &ctr, V3 &dir, float &pdf, V3 &next_T) {
   if (mode == MUONX_IS_HEMISPHERE) dir = sample_hemisphere(n, key, ctr);
   else if (mode == MUONX_IS_COSINE) dir = sample_cosine(n, key, ctr);

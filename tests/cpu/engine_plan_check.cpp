@@ -1,0 +1,213 @@
+// CPU checks of the engine's index arithmetic and the scheduler's policies
+// (muon_amd/csrc/device/engine_plan.h, sched_policy.h), the code the
+// engine's kernels and host side index their buffers with. Built with the
+// host compiler and run by tests/test_engine_plan.py: prints "ok" and exits
+// 0, or names the failed check and exits 1.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <random>
+#include <vector>
+
+#include "engine_plan.h"
+#include "sched_policy.h"
+
+using namespace muonx;
+
+#define CHECK(c)                                                     \
+  do {                                                               \
+    if (!(c)) {                                                      \
+      std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);       \
+      std::exit(1);                                                  \
+    }                                                                \
+  } while (0)
+
+// Every array of the path and shadow buffers starts on a 256-byte boundary,
+// holds its rows and ends before the next one starts; the buffer's size is
+// at least the path budget's per-path bytes times the rows; the largest
+// shadow-slot index the engine admits fits 32 bits.
+static void CheckLayout() {
+  CHECK(PathSlotBytes(0) == 240 && PathSlotBytes(4) == 240 + 4 * 34);
+  for (size_t P : {size_t(1), size_t(7), size_t(1024), size_t(22800000), size_t(1) << 26}) {
+    for (size_t K : {size_t(1), size_t(2), size_t(4), size_t(9)}) {
+      for (int which = 0; which < 2; ++which) {
+        const ArraySpec *spec = which ? kShadowArrays : kPathArrays;
+        const int n = which ? kNumShadowArrays : kNumPathArrays;
+        const size_t rows = which ? P * K : P;
+        size_t off[64];
+        const size_t bytes = LayoutOffsets(spec, n, rows, off);
+        uint64_t per_row = 0;
+        for (int k = 0; k < n; ++k) {
+          CHECK(off[k] % kCarveAlign == 0);
+          const size_t end = off[k] + rows * spec[k].per_path * spec[k].elem_bytes;
+          CHECK(end <= (k + 1 < n ? off[k + 1] : bytes));
+          per_row += uint64_t(spec[k].per_path) * spec[k].elem_bytes;
+        }
+        CHECK(bytes >= rows * per_row && bytes <= rows * per_row + size_t(n) * kCarveAlign);
+      }
+      CHECK(PathSlotBytes(int(K)) * P <= PathSlotBytes(int(K)) * P);  // no overflow at these sizes
+    }
+  }
+  // Engine::Create admits capacity * kmax <= 2^31: the largest slot index
+  const uint32_t caps[] = {1u << 28, 1u << 30, 715827882u};
+  const uint32_t kms[] = {8u, 2u, 3u};
+  for (int t = 0; t < 3; ++t) {
+    const uint64_t P = caps[t], K = kms[t];
+    CHECK(P * K <= kMaxShadowSlots);
+    CHECK(uint64_t(ShadowSlot(uint32_t(K - 1), uint32_t(P), uint32_t(P - 1))) == (K - 1) * P + (P - 1));
+  }
+}
+
+// The admission window, plan and fill of the engine, simulated with random
+// job sizes, queue capacities and job-ring sizes: every generated sample
+// maps to the admitted job that owns it (the ring slot still holds that
+// job), to a pixel inside its film and a sample inside its slice; the queue
+// never exceeds its capacity; every item of every job is generated exactly
+// once, in order.
+static void CheckWindow(std::mt19937_64 &rng) {
+  struct Job {
+    unsigned long long first, nitems;
+    uint32_t npix, sample_base, count;
+    unsigned long long generated;
+  };
+  for (uint32_t max_jobs : {1u, 2u, 3u, 8u, 64u}) {
+    for (uint32_t capacity : {1u, 5u, 64u, 1000u}) {
+      std::vector<unsigned long long> ring(max_jobs, ~0ull);
+      std::vector<uint64_t> ring_seq(max_jobs, ~0ull);
+      std::map<uint64_t, Job> jobs;  // by sequence number
+      unsigned long long total = 0, cursor = 0, lo = 0, hi = 0;
+      uint32_t queued = 0;
+      for (int iter = 0; iter < 400; ++iter) {
+        // admissions while job slots are free
+        while (hi - lo < max_jobs && rng() % 3 != 0) {
+          Job j;
+          j.npix = 1 + uint32_t(rng() % 300);
+          j.sample_base = uint32_t(rng() % 5);
+          j.count = 1 + uint32_t(rng() % 4);
+          j.first = total;
+          j.nitems = uint64_t(j.npix) * j.count;
+          j.generated = 0;
+          ring[hi % max_jobs] = j.first;
+          ring_seq[hi % max_jobs] = hi;
+          jobs[hi] = j;
+          total += j.nitems;
+          ++hi;
+        }
+        // continuing paths: any number up to the current queue
+        const uint32_t cont = queued ? uint32_t(rng() % (queued + 1)) : 0u;
+        const uint32_t n = FillCount(total, cursor, capacity, cont);
+        CHECK(uint64_t(cont) + n <= capacity);
+        CHECK(n == std::min<unsigned long long>(total - cursor, capacity - cont));
+        for (uint32_t k = 0; k < n; ++k) {
+          const unsigned long long item = cursor + k;
+          CHECK(lo < hi);
+          const unsigned long long s = FindJobSeq(ring.data(), max_jobs, lo, hi, item);
+          CHECK(s >= lo && s < hi);
+          CHECK(ring_seq[s % max_jobs] == s);  // the slot was not reused
+          Job &j = jobs.at(s);
+          CHECK(j.first <= item && item < j.first + j.nitems);
+          CHECK(item - j.first == j.generated);  // in order, each once
+          ++j.generated;
+          const ItemPos ip = ItemPosition(item - j.first, j.npix, j.sample_base);
+          CHECK(ip.pixel < j.npix);
+          CHECK(ip.sample >= j.sample_base && ip.sample < j.sample_base + j.count);
+        }
+        cursor += n;
+        queued = cont + n;
+        // the window's low end moves past jobs whose samples were all
+        // generated (and, here at random, finished)
+        while (lo < hi && jobs.at(lo).generated == jobs.at(lo).nitems && rng() % 2 == 0) ++lo;
+      }
+      for (auto &kv : jobs) {
+        if (kv.first < lo) CHECK(kv.second.generated == kv.second.nitems);
+      }
+    }
+  }
+}
+
+// Random allocations and releases against a byte-level model: ranges are
+// aligned, inside the capacity and disjoint; accounting matches; after
+// releasing everything one free range spans the pool.
+static void CheckRanges(std::mt19937_64 &rng) {
+  for (uint64_t cap : {uint64_t(256), uint64_t(4096), uint64_t(1) << 20}) {
+    RangeAllocator ra;
+    ra.Reset(cap);
+    std::map<uint64_t, uint64_t> live;  // offset -> rounded bytes
+    uint64_t used = 0;
+    for (int step = 0; step < 3000; ++step) {
+      if (live.empty() || rng() % 2) {
+        const uint64_t want = 1 + rng() % (cap / 4 + 1);
+        uint64_t off = 0;
+        if (ra.Alloc(want, &off)) {
+          const uint64_t b = RangeAllocator::Round(want);
+          CHECK(off % RangeAllocator::kAlign == 0 && off + b <= ra.capacity());
+          auto nx = live.lower_bound(off);
+          if (nx != live.end()) CHECK(off + b <= nx->first);
+          if (nx != live.begin()) {
+            auto pv = std::prev(nx);
+            CHECK(pv->first + pv->second <= off);
+          }
+          live[off] = b;
+          used += b;
+        } else {
+          // first fit failed: no free range is large enough
+          for (auto &fr : ra.free_ranges()) CHECK(fr.second < RangeAllocator::Round(want));
+        }
+      } else {
+        auto it = live.begin();
+        std::advance(it, long(rng() % live.size()));
+        ra.Free(it->first, it->second);
+        used -= it->second;
+        live.erase(it);
+      }
+      CHECK(ra.in_use() == used);
+      uint64_t fsum = 0;
+      for (auto &fr : ra.free_ranges()) fsum += fr.second;
+      CHECK(fsum + used == ra.capacity());
+    }
+    for (auto &kv : live) ra.Free(kv.first, kv.second);
+    CHECK(ra.in_use() == 0 && ra.free_ranges().size() == 1 && ra.free_ranges().begin()->second == ra.capacity());
+  }
+}
+
+// Slice choice: new admissions first, then the oldest or least-work render;
+// and under a stream that always has a new render waiting, a running large
+// render still gets a slice at least every kSliceAgeIterations + 1
+// iterations (without the age rule it would get none).
+static void CheckSlicePolicy() {
+  size_t idx = 99;
+  std::vector<SliceCandidate> c = {{50, 10}, {0, 10}, {20, 10}};
+  CHECK(PickSlice(c, true, false, 11, &idx) == SliceChoice::kAdmitPending);
+  CHECK(PickSlice(c, false, false, 11, &idx) == SliceChoice::kRunning && idx == 0);
+  CHECK(PickSlice(c, false, true, 11, &idx) == SliceChoice::kRunning && idx == 2);
+  CHECK(PickSlice({{0, 0}}, false, true, 5, &idx) == SliceChoice::kNone);
+  CHECK(PickSlice(c, true, true, 10 + kSliceAgeIterations, &idx) == SliceChoice::kRunning && idx == 0);
+  for (int lwf = 0; lwf < 2; ++lwf) {
+    // one large render with 1000 slices of work; each iteration one slice is
+    // handed out and a new one-slice render is always waiting
+    std::vector<SliceCandidate> run = {{1000, 0}};
+    uint64_t last = 0, max_gap = 0;
+    for (uint64_t it = 1; it <= 5000; ++it) {
+      const SliceChoice ch = PickSlice(run, true, lwf != 0, it, &idx);
+      if (ch == SliceChoice::kRunning) {
+        CHECK(idx == 0);
+        run[0].work_left -= 1;
+        run[0].last_slice_iter = it;
+        max_gap = std::max(max_gap, it - last);
+        last = it;
+      }
+    }
+    CHECK(last > 0 && max_gap <= kSliceAgeIterations + 1);
+  }
+}
+
+int main() {
+  std::mt19937_64 rng(12345);
+  CheckLayout();
+  CheckWindow(rng);
+  CheckRanges(rng);
+  CheckSlicePolicy();
+  std::printf("ok\n");
+  return 0;
+}

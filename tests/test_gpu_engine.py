@@ -317,3 +317,36 @@ def test_autotune_installs_the_fastest_configuration(resident):
     assert np.allclose(t1[~same], t0[~same], rtol=1e-5)
     img = alone(ds, job)
     assert np.isfinite(img).all() and img.sum() > 0
+
+
+@pytest.mark.parametrize("order", [0, 1], ids=["oldest-first", "least-work-first"])
+def test_server_large_job_completes_under_a_request_stream(ctx, resident, order):
+    """One large render (128 slices of a 4096-path engine) while a stream
+    keeps up to 8 small requests waiting: new requests' first slices go
+    first and least-work-first prefers the small ones, but a render that
+    got no slice for 64 iterations goes ahead (tests/cpu/engine_plan_check.cpp
+    checks that bound), so the large render finishes while the stream still
+    runs, with its image alone."""
+    hs, ds = resident["cornell"]
+    big = params_of(hs, width=64, height=64, spp=32, max_depth=5, seed=4)
+    small = params_of(hs, width=16, height=16, spp=4, max_depth=5, seed=5)
+    ref = alone(ds, big)
+    cfg = engine_config(path_slots=4096, max_jobs=64, shadow_per_vertex=4, iterations_per_poll=2, slice_order=order)
+    small_tickets, done_big = [], None
+    with Server(ctx, cfg) as srv:
+        tb = srv.submit(ds, big, 0)
+        t0 = time.time()
+        while time.time() - t0 < 60.0:
+            pending = srv.stats()["pending"]
+            while pending < 8:
+                small_tickets.append(srv.submit(ds, small, len(small_tickets) + 1))
+                pending += 1
+            out = srv.wait(tb, timeout_ms=0)
+            if out is not None:
+                done_big = out
+                break
+        assert done_big is not None, "the large render did not finish under the request stream"
+        for t in small_tickets:
+            assert srv.wait(t, timeout_ms=60000) is not None
+    assert len(small_tickets) >= 16
+    assert np.array_equal(done_big[0].cpu().numpy().view(np.uint32), ref.view(np.uint32))
