@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -303,12 +304,24 @@ class PlyReader {
     }
     return true;
   }
+  // Bytes left in the body: every value takes at least one (a binary
+  // scalar, or an ASCII token with its separator), so a count above this
+  // cannot be read and marks a truncated or corrupt file.
+  size_t Remaining() const { return p_ < d_.size() ? d_.size() - p_ : 0; }
 
  private:
   const std::string &d_;
   size_t p_;
   int fmt_;  // 0 ascii, 1 little endian, 2 big endian
 };
+
+// A list count or vertex index read as a double: accepted when it is finite,
+// integral and within [0, limit].
+bool PlyIndex(double v, double limit, long &out) {
+  if (!(v >= 0.0 && v <= limit) || v != std::floor(v)) return false;
+  out = long(v);
+  return true;
+}
 
 bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string &err) {
   std::istringstream hs(data);
@@ -350,7 +363,8 @@ bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string
         p.list = true;
         p.count_type = PlyTypeOf(ct);
         p.type = PlyTypeOf(it);
-        if (p.count_type == PlyType::kBad || p.type == PlyType::kBad) {
+        if (p.count_type == PlyType::kBad || p.type == PlyType::kBad || p.count_type == PlyType::kF32 ||
+            p.count_type == PlyType::kF64) {
           err = "ply: bad list property types";
           return false;
         }
@@ -409,6 +423,11 @@ bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string
       vstride = 0;
       for (const PlyProp &p : e.props) vstride += p.list ? 0 : 1;
     }
+    if (e.props.empty()) continue;  // nothing to read, whatever the declared count
+    if (e.count > rd.Remaining()) {
+      err = "ply: element '" + e.name + "' declares more rows than the body holds";
+      return false;
+    }
     for (size_t r = 0; r < e.count; ++r) {
       std::array<float, 3> pos{};
       for (size_t k = 0; k < e.props.size(); ++k) {
@@ -416,16 +435,24 @@ bool ParsePly(const std::string &data, std::vector<LoadedMesh> &out, std::string
         double v;
         if (p.list) {
           double cnt;
-          if (!rd.Read(p.count_type, cnt) || cnt < 0) {
-            err = "ply: truncated list";
+          long n = 0;
+          if (!rd.Read(p.count_type, cnt) || !PlyIndex(cnt, double(rd.Remaining()), n)) {
+            err = "ply: truncated list or bad list count";
             return false;
           }
-          for (long j = 0; j < long(cnt); ++j) {
+          for (long j = 0; j < n; ++j) {
             if (!rd.Read(p.type, v)) {
               err = "ply: truncated list";
               return false;
             }
-            if (is_face && int(k) == ifl) fidx.push_back(long(v));
+            if (is_face && int(k) == ifl) {
+              long idx;
+              if (!PlyIndex(v, 2147483647.0, idx)) {
+                err = "ply: face index is not a non-negative integer";
+                return false;
+              }
+              fidx.push_back(idx);
+            }
           }
           continue;
         }

@@ -256,3 +256,41 @@ def test_ply_truncated_body_is_a_warning(tmp_path):
     (tmp_path / "s.muon").write_text(HEAD + "load t.ply\n" + TAIL)
     hs = HostScene.from_file(str(tmp_path / "s.muon"))
     assert hs.warnings == 1 and hs.desc.num_prims == 1
+
+
+PLY_VERTS = "element vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+PLY_V_ASCII = "0 0 0\n1 0 0\n0 1 0\n"
+
+
+@pytest.mark.parametrize("header,body", [
+    # float list counts are refused at the header
+    ("element face 1\nproperty list float int vertex_indices\n", "3 0 1 2\n"),
+    # non-integral, non-finite, negative and huge face indices
+    ("element face 1\nproperty list uchar float vertex_indices\n", "3 0 1 1.5\n"),
+    ("element face 1\nproperty list uchar double vertex_indices\n", "3 0 1 nan\n"),
+    ("element face 1\nproperty list uchar double vertex_indices\n", "3 0 1 -1\n"),
+    ("element face 1\nproperty list uchar double vertex_indices\n", "3 0 1 1e30\n"),
+    # a list count beyond the body
+    ("element face 1\nproperty list uint int vertex_indices\n", "4000000000 0 1 2\n"),
+    # an element declaring more rows than the body can hold
+    ("element face 99999999999\nproperty list uchar int vertex_indices\n", "3 0 1 2\n"),
+], ids=["float-count", "fractional-index", "nan-index", "negative-index", "huge-index", "huge-count", "huge-rows"])
+def test_ply_bad_counts_and_indices_are_warnings(tmp_path, header, body):
+    """Malformed counts and indices reject the file (a warning; the scene's
+    other primitives stay) instead of being cast unchecked."""
+    data = "ply\nformat ascii 1.0\n" + PLY_VERTS + header + "end_header\n" + PLY_V_ASCII + body
+    (tmp_path / "t.ply").write_text(data)
+    (tmp_path / "s.muon").write_text(HEAD + "load t.ply\n" + TAIL)
+    hs = HostScene.from_file(str(tmp_path / "s.muon"))
+    assert hs.warnings == 1 and hs.desc.num_prims == 1
+
+
+def test_ply_property_less_element_with_huge_count_is_skipped(tmp_path):
+    """An element without properties reads nothing whatever its declared
+    count (it used to loop over the count)."""
+    data = ("ply\nformat ascii 1.0\nelement junk 18446744073709551615\n" + PLY_VERTS +
+            "element face 1\nproperty list uchar int vertex_indices\nend_header\n" + PLY_V_ASCII + "3 0 1 2\n")
+    (tmp_path / "t.ply").write_text(data)
+    (tmp_path / "s.muon").write_text(HEAD + "load t.ply\n" + TAIL)
+    hs = HostScene.from_file(str(tmp_path / "s.muon"))
+    assert hs.warnings == 0 and hs.desc.num_prims == 2
