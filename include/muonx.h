@@ -140,9 +140,12 @@ typedef struct muonx_trace_stats {
                            direction misses every emitting primitive, so their closest hit
                            cannot be a light (path tracer; 0 for the other integrators) */
   /* TraceStats' test counters (muon/stats.h:11-45, counted at
-   * muon/acceleration.cc:92-114,162-179): counted by the deterministic
-   * integrators, whose exact traversal visits the reference's nodes, so they
-   * equal the reference's; 0 for the path tracer, which walks a 4-wide tree. */
+   * muon/acceleration.cc:92-114,162-179). The deterministic integrators'
+   * exact traversal visits the reference's nodes, so theirs equal the
+   * reference's. The path tracer walks the fast mode's 4-wide tree: with
+   * tuning.count_work its counters are that traversal's work (child boxes
+   * tested and passed, primitive tests and hits), not the reference's
+   * binary-tree counts; without it they are 0. */
   uint64_t object_tests;
   uint64_t object_hits;
   uint64_t bounds_tests;
@@ -207,6 +210,8 @@ typedef struct muonx_tuning {
   int32_t ray_fetch;  /* fast-mode queries: 0 = one ray per lane per grid step; 1..64 = a lane whose
                          ray is done takes a new one once that many lanes of its wavefront are idle */
   int32_t node_bits;  /* 4-wide nodes: 32 = float child boxes (128 B), 8 = 8-bit quantized boxes (64 B) */
+  int32_t count_work; /* path tracer: 1 = count its traversals' box and primitive tests and hits into
+                         muonx_trace_stats (about 3 % slower), 0 = leave those counters at 0 (default) */
 } muonx_tuning;
 int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t);
 int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t);

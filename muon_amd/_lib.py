@@ -58,7 +58,7 @@ class Query(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("stack_entries", C.c_int32), ("while_while", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("path_budget_mb", C.c_int32), ("bvh_width", C.c_int32), ("ray_fetch", C.c_int32),
-                ("node_bits", C.c_int32)]
+                ("node_bits", C.c_int32), ("count_work", C.c_int32)]
 
 
 class AutotuneReport(C.Structure):

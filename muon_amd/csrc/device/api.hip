@@ -73,9 +73,10 @@ int muonx_ctx_set_tuning(muonx_ctx *ctx, const muonx_tuning *t) {
   }
   if (!(t->stack_entries == 0 || t->stack_entries == 8 || t->stack_entries == 16) || t->blocks_per_cu < 1 ||
       t->blocks_per_cu > 64 || t->path_budget_mb < 16 || !(t->bvh_width == 2 || t->bvh_width == 4) ||
-      t->ray_fetch < 0 || t->ray_fetch > 64 || !(t->node_bits == 8 || t->node_bits == 32)) {
+      t->ray_fetch < 0 || t->ray_fetch > 64 || !(t->node_bits == 8 || t->node_bits == 32) ||
+      !(t->count_work == 0 || t->count_work == 1)) {
     SetError("muonx_ctx_set_tuning: stack_entries must be 0, 8 or 16; blocks_per_cu 1..64; path_budget_mb >= 16; "
-             "bvh_width 2 or 4; ray_fetch 0..64; node_bits 8 or 32");
+             "bvh_width 2 or 4; ray_fetch 0..64; node_bits 8 or 32; count_work 0 or 1");
     return MUONX_ERR_INVALID;
   }
   std::lock_guard<std::mutex> lock(ctx->mu);

@@ -41,10 +41,12 @@ constexpr int kShadeWaves = kShadeBlock / 64;
 constexpr int kMaxBounces = 1024;  // hard cap for max_depth = -1 without Russian roulette
 constexpr double kFilmScale = 16777216.0;  // fixed-point units per radiance unit
 
-// Ray counters of the engine: primary, secondary (continuation and MIS rays,
-// as the reference counts them), shadow, and the MIS rays among the
-// secondary ones resolved without a traversal.
-constexpr int kRayCounters = 4;
+// Counters of the engine: primary, secondary (continuation and MIS rays, as
+// the reference counts them), shadow rays, the MIS rays among the secondary
+// ones resolved without a traversal, and the traversals' work in the device
+// tree: primitive tests and hits, child boxes tested and passed.
+constexpr int kRayCounters = kEngineCounters;
+constexpr int kCtrObjTests = 4, kCtrObjHits = 5, kCtrBoxTests = 6, kCtrBoxHits = 7;
 
 struct JobDev {
   CamParams cam;
@@ -85,7 +87,7 @@ struct EngCounters {
   unsigned long long cursor;          // next work item to generate
   unsigned long long total_items;     // admitted work (host-written)
   unsigned long long seq_lo, seq_hi;  // admitted jobs (host-written)
-  unsigned long long rays[kRayCounters];  // primary, secondary, shadow, culled MIS
+  unsigned long long rays[kRayCounters];  // primary, secondary, shadow, culled MIS, traversal work
   // This iteration's counts by kernel block (block % kRaySlots), folded into
   // `rays` by the plan kernel: one counter updated by every wavefront of a
   // launch would serialise its atomics.

@@ -18,21 +18,30 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
   return x < 4.0e18 ? (unsigned long long)x : 4000000000000000000ull;
 }
 
-// Adds a wavefront's count to this block's slot of ray counter `id`.
+// Adds a wavefront's count to this block's slot of counter `id`.
 __device__ __forceinline__ void ray_count(EngCounters *c, int id, uint32_t n) {
   unsigned long long v = n;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
   if (__lane_id() == 0 && v) atomicAdd(&c->ray_slots[id][blockIdx.x % kRaySlots].v, v);
 }
 
+// Adds a wavefront's traversal work (counted per lane) to the counters.
+__device__ __forceinline__ void work_count(EngCounters *c, const TraceCounters &t) {
+  ray_count(c, kCtrObjTests, t.prims);
+  ray_count(c, kCtrObjHits, t.ohits);
+  ray_count(c, kCtrBoxTests, t.btests);
+  ray_count(c, kCtrBoxHits, t.bhits);
+}
+
 // ---------------------------------------------------------------- extend
-template <int KIND, bool WW, int WIDTH>
+template <int KIND, bool WW, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
   const PathState &ps = a.st[cur];
   uint32_t prim_rays = 0, sec_rays = 0;
+  TraceCounters c;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t i = base + threadIdx.x;
     if (i < n) {
@@ -47,9 +56,8 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
         const DevSceneView sv = a.scenes[a.jobs[js].scene];
         const TraceRay r = make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z});
         float best = __builtin_inff();
-        TraceCounters c;
         auto st = make_stack<KIND>(lds, kEngBlock, sa);
-        hit = trace_fast<WIDTH, false, false, WW>(sv, r, best, st, c);
+        hit = trace_fast<WIDTH, false, COUNT, WW>(sv, r, best, st, c);
         if (depth == 0) ++prim_rays;
         else ++sec_rays;
       }
@@ -58,6 +66,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
   }
   ray_count(a.c, 0, prim_rays);
   ray_count(a.c, 1, sec_rays);
+  if (COUNT) work_count(a.c, c);
 }
 
 // ---------------------------------------------------------------- shade
@@ -316,13 +325,14 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
 }
 
 // ---------------------------------------------------------------- shadow (any hit)
-template <int KIND, bool WW, int WIDTH>
+template <int KIND, bool WW, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t P = a.capacity;
   const uint32_t n = a.c->shadow_count.v;
   const PathState &ps = a.st[cur];
   uint32_t traced = 0;
+  TraceCounters c;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t j = base + threadIdx.x;
     if (j < n) {
@@ -332,23 +342,24 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs s
       const float4 so = a.shift[i], sd = a.s_dir[sl];
       const TraceRay r = make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z});
       float best = sd.w;
-      TraceCounters c;
       auto st = make_stack<KIND>(lds, kEngBlock, sa);
-      const int32_t hit = trace_fast<WIDTH, true, false, WW>(sv, r, best, st, c);
+      const int32_t hit = trace_fast<WIDTH, true, COUNT, WW>(sv, r, best, st, c);
       a.s_vis[sl] = hit < 0 ? 1 : 0;
       ++traced;
     }
   }
   ray_count(a.c, 2, traced);
+  if (COUNT) work_count(a.c, c);
 }
 
 // ---------------------------------------------------------------- MIS extend
-template <int KIND, bool WW, int WIDTH>
+template <int KIND, bool WW, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, int cur) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t n = a.c->mis_count.v;
   const PathState &ps = a.st[cur];
   uint32_t traced = 0;
+  TraceCounters c;
   for (uint32_t base = blockIdx.x * kEngBlock; base < n; base += gridDim.x * kEngBlock) {
     const uint32_t j = base + threadIdx.x;
     if (j < n) {
@@ -357,27 +368,27 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, 
       const DevSceneView s = a.scenes[a.jobs[ps.job[i]].scene];
       const TraceRay r = make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z});
       float best = md.w;  // the nearest emitter hit, or +inf
-      TraceCounters c;
       auto st = make_stack<KIND>(lds, kEngBlock, sa);
-      a.m_hit[i] = trace_fast<WIDTH, false, false, WW>(s, r, best, st, c);
+      a.m_hit[i] = trace_fast<WIDTH, false, COUNT, WW>(s, r, best, st, c);
       ++traced;
     }
   }
   ray_count(a.c, 1, traced);
+  if (COUNT) work_count(a.c, c);
 }
 
 // ---------------------------------------------------------------- per-lane fetch variants
 // The three traversal kernels with per-lane work claiming (WaveFetch): a
 // lane that finishes its ray takes the next one once `fetch_min` lanes of its
 // wavefront are idle. Results are those of the kernels above.
-template <int KIND, int WIDTH>
+template <int KIND, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) k_eng_extend_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t n = a.c->qcount[cur].v;
   const float4 *q = a.q_ray[cur];
   const PathState &ps = a.st[cur];
   auto st = make_stack<KIND>(lds, kEngBlock, sa);
-  LaneTrace<WIDTH, false, false> lt;
+  LaneTrace<WIDTH, false, COUNT> lt;
   DevSceneView sv;
   TraceCounters c;
   WaveFetch wf(n);
@@ -417,16 +428,17 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
   }
   ray_count(a.c, 0, prim_rays);
   ray_count(a.c, 1, sec_rays);
+  if (COUNT) work_count(a.c, c);
 }
 
-template <int KIND, int WIDTH>
+template <int KIND, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) k_eng_shadow_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t P = a.capacity;
   const uint32_t n = a.c->shadow_count.v;
   const PathState &ps = a.st[cur];
   auto st = make_stack<KIND>(lds, kEngBlock, sa);
-  LaneTrace<WIDTH, true, false> lt;
+  LaneTrace<WIDTH, true, COUNT> lt;
   DevSceneView sv;
   TraceCounters c;
   WaveFetch wf(n);
@@ -458,15 +470,16 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
     }
   }
   ray_count(a.c, 2, traced);
+  if (COUNT) work_count(a.c, c);
 }
 
-template <int KIND, int WIDTH>
+template <int KIND, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) k_eng_mis_dyn(EngArgs a, StackArgs sa, int cur, int fetch_min) {
   __shared__ uint2 lds[StackOf<KIND>::kLds * kEngBlock];
   const uint32_t n = a.c->mis_count.v;
   const PathState &ps = a.st[cur];
   auto st = make_stack<KIND>(lds, kEngBlock, sa);
-  LaneTrace<WIDTH, false, false> lt;
+  LaneTrace<WIDTH, false, COUNT> lt;
   DevSceneView sv;
   TraceCounters c;
   WaveFetch wf(n);
@@ -497,6 +510,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
     }
   }
   ray_count(a.c, 1, traced);
+  if (COUNT) work_count(a.c, c);
 }
 
 // ---------------------------------------------------------------- retire
@@ -728,25 +742,31 @@ struct IterLaunch {
   const EngArgs &a;
   const TraverseCfg &cfg;
   int cur;
-  int fetch;  // per-lane fetch threshold, 0 = grid-stride kernels
+  int fetch;   // per-lane fetch threshold, 0 = grid-stride kernels
+  bool count;  // traversal kernels count their box and primitive tests
   hipStream_t st;
   template <int KIND, bool WW, int WIDTH>
   void run() const {
+    if (count) launch<KIND, WW, WIDTH, true>();
+    else launch<KIND, WW, WIDTH, false>();
+  }
+  template <int KIND, bool WW, int WIDTH, bool COUNT>
+  void launch() const {
     const StackArgs sa{cfg.spill, cfg.gcap};
     const dim3 grid(cfg.grid), block(kEngBlock);
     const dim3 sgrid(std::max(1, cfg.grid * kEngBlock / kShadeBlock)), sblock(kShadeBlock);
     if (fetch > 0) {
-      hipLaunchKernelGGL((k_eng_extend_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
+      hipLaunchKernelGGL((k_eng_extend_dyn<KIND, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur, fetch);
       hipLaunchKernelGGL(k_eng_shade_light, sgrid, sblock, 0, st, a, cur);
       hipLaunchKernelGGL(k_eng_shade_bsdf, sgrid, sblock, 0, st, a, cur);
-      hipLaunchKernelGGL((k_eng_shadow_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
-      hipLaunchKernelGGL((k_eng_mis_dyn<KIND, WIDTH>), grid, block, 0, st, a, sa, cur, fetch);
+      hipLaunchKernelGGL((k_eng_shadow_dyn<KIND, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur, fetch);
+      hipLaunchKernelGGL((k_eng_mis_dyn<KIND, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur, fetch);
     } else {
-      hipLaunchKernelGGL((k_eng_extend<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
+      hipLaunchKernelGGL((k_eng_extend<KIND, WW, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur);
       hipLaunchKernelGGL(k_eng_shade_light, sgrid, sblock, 0, st, a, cur);
       hipLaunchKernelGGL(k_eng_shade_bsdf, sgrid, sblock, 0, st, a, cur);
-      hipLaunchKernelGGL((k_eng_shadow<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
-      hipLaunchKernelGGL((k_eng_mis<KIND, WW, WIDTH>), grid, block, 0, st, a, sa, cur);
+      hipLaunchKernelGGL((k_eng_shadow<KIND, WW, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur);
+      hipLaunchKernelGGL((k_eng_mis<KIND, WW, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur);
     }
     hipLaunchKernelGGL(k_eng_retire, grid, block, 0, st, a, cur);
     hipLaunchKernelGGL(k_eng_plan, dim3(1), dim3(64), 0, st, a, cur);
@@ -1000,7 +1020,7 @@ int Engine::SyncWindow() {
 }
 
 int Engine::LaunchIteration(int cur) {
-  DispatchTraverse(cfg_, IterLaunch{a_, cfg_, cur, tuning_.ray_fetch, stream_});
+  DispatchTraverse(cfg_, IterLaunch{a_, cfg_, cur, tuning_.ray_fetch, tuning_.count_work != 0, stream_});
   MX_HIP(hipGetLastError());
   return MUONX_OK;
 }

@@ -142,7 +142,7 @@ extern "C" int muonx_render_batch(muonx_ctx *ctx, const muonx_job *jobs, uint32_
   int rc = ContextScheduler(ctx, cfg, jobs, n, &s);
   if (rc != MUONX_OK) return rc;
   const uint64_t t0 = NowNs();
-  unsigned long long rays[kRayCounters] = {0, 0, 0, 0};
+  unsigned long long rays[kRayCounters] = {};
   rc = RunInline(s, jobs, n, results, rays);
   if (rc != MUONX_OK) return rc;
   if (stats) {
@@ -151,7 +151,10 @@ extern "C" int muonx_render_batch(muonx_ctx *ctx, const muonx_job *jobs, uint32_
     stats->shadow_rays = rays[2];
     stats->culled_rays = rays[3];
     stats->render_ns = NowNs() - t0;
-    stats->object_tests = stats->object_hits = stats->bounds_tests = stats->bounds_hits = 0;
+    stats->object_tests = rays[kCtrObjTests];
+    stats->object_hits = rays[kCtrObjHits];
+    stats->bounds_tests = rays[kCtrBoxTests];
+    stats->bounds_hits = rays[kCtrBoxHits];
   }
   return MUONX_OK;
 }

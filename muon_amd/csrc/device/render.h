@@ -75,11 +75,14 @@ struct RenderWorkspace {
 RenderWorkspace *GetWorkspace(muonx_ctx *ctx);
 void DestroyContextScheduler(RenderWorkspace *ws);
 CamParams MakeCam(const muonx_render_params &p);
+// Counters of the path-tracer engine: primary, secondary, shadow and culled
+// MIS rays, then the traversals' primitive tests and hits and child boxes
+// tested and passed (in the device tree).
+constexpr int kEngineCounters = 8;
 // One path-tracer job on the context's engine, ordered after earlier work on
-// `stream` (later work on `stream` waits for the film). rays: primary,
-// secondary, shadow.
+// `stream` (later work on `stream` waits for the film); its counters in rays.
 int RenderPathTracer(muonx_scene *scene, const muonx_render_params &p, float *accum, hipStream_t stream,
-                     unsigned long long rays[4]);  // primary, secondary, shadow, culled MIS
+                     unsigned long long rays[kEngineCounters]);
 
 }  // namespace muonx
 

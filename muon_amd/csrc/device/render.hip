@@ -309,7 +309,7 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
   int rc;
   unsigned long long *ctr = ws->counters.as<unsigned long long>();
   bool host_rays = false;
-  unsigned long long rays[4] = {0, 0, 0, 0};
+  unsigned long long rays[kEngineCounters] = {};
   switch (p.integrator) {
     case MUONX_INTEGRATOR_NORMALS:
     case MUONX_INTEGRATOR_ALBEDO:
@@ -345,10 +345,11 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
     stats->shadow_rays = host_rays ? rays[2] : h[kCtrShadow];
     stats->render_ns = uint64_t(double(ms) * 1e6);
     stats->culled_rays = host_rays ? rays[3] : 0;
-    stats->object_tests = host_rays ? 0 : h[kCtrObjectTests];
-    stats->object_hits = host_rays ? 0 : h[kCtrObjectHits];
-    stats->bounds_tests = host_rays ? 0 : h[kCtrBoundsTests];
-    stats->bounds_hits = host_rays ? 0 : h[kCtrBoundsHits];
+    // the path tracer's: the work of its traversals in the device tree
+    stats->object_tests = host_rays ? rays[4] : h[kCtrObjectTests];
+    stats->object_hits = host_rays ? rays[5] : h[kCtrObjectHits];
+    stats->bounds_tests = host_rays ? rays[6] : h[kCtrBoundsTests];
+    stats->bounds_hits = host_rays ? rays[7] : h[kCtrBoundsHits];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -71,7 +71,7 @@ struct muonx_ctx {
   std::mutex mu;
   muonx::RenderWorkspace *ws = nullptr;
   int num_cus = 0;
-  muonx_tuning tuning{8, 1, 16, 8192, 4, 48, 8};  // best of the sweeps on the level-7 icosphere
+  muonx_tuning tuning{8, 1, 16, 8192, 4, 48, 8, 0};  // best of the sweeps on the level-7 icosphere
   muonx::DevBuf spill;      // per-thread traversal stack spill region
   muonx::DevBuf fetch_ctr;  // ray claim counter of the per-lane fetch kernels
   // Orders kernels that share `spill` when they are issued on different streams.

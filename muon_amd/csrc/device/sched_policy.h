@@ -66,26 +66,28 @@ class RangeAllocator {
 
 // A running render as the slice choice sees it (in arrival order).
 struct SliceCandidate {
-  uint64_t work_left;        // pixels x samples not yet handed to the engine (0: nothing left)
-  uint64_t last_slice_iter;  // engine iteration at which it last got a slice
+  uint64_t work_left;  // pixels x samples not yet handed to the engine (0: nothing left)
+  uint32_t passes;     // slices given to newer work since its last slice, while it had work left
 };
 
-// A running render that got no slice for this many engine iterations goes
-// ahead of new admissions and of the slice order, so that a steady stream
-// of new or small requests cannot starve it.
-constexpr uint64_t kSliceAgeIterations = 64;
+// A running render passed over this many times by newer work (a newly
+// admitted render, or a later render chosen by least-work-first) goes first,
+// so that no stream of new or smaller requests can starve it. Under
+// oldest-first it is passed over only by new admissions, which a render
+// sees a few of in its lifetime at ordinary loads.
+constexpr uint32_t kSlicePassLimit = 16;
 
 enum class SliceChoice { kAdmitPending, kRunning, kNone };
 
 // Chooses what the next slice of the engine's backlog is for: the oldest
-// running render that has waited kSliceAgeIterations or more (`*index`);
-// else a pending render whose film fits (its first slice); else a running
-// render with work left, the oldest (least_work_first false) or the one
-// with the least work left.
+// running render passed over kSlicePassLimit times (`*index`); else a
+// pending render whose film fits (its first slice); else a running render
+// with work left, the oldest (least_work_first false) or the one with the
+// least work left.
 inline SliceChoice PickSlice(const std::vector<SliceCandidate> &running, bool pending_fits, bool least_work_first,
-                             uint64_t now_iter, size_t *index) {
+                             size_t *index) {
   for (size_t k = 0; k < running.size(); ++k) {
-    if (running[k].work_left && now_iter - running[k].last_slice_iter >= kSliceAgeIterations) {
+    if (running[k].work_left && running[k].passes >= kSlicePassLimit) {
       *index = k;
       return SliceChoice::kRunning;
     }
@@ -104,6 +106,18 @@ inline SliceChoice PickSlice(const std::vector<SliceCandidate> &running, bool pe
     }
   }
   return found ? SliceChoice::kRunning : SliceChoice::kNone;
+}
+
+// Updates the pass counts after a choice: every render older than the
+// chosen one (all of them when a new render was admitted) that had work left
+// was passed over; the chosen render's count restarts.
+inline void RecordPick(std::vector<SliceCandidate> &running, SliceChoice choice, size_t index) {
+  const size_t older = choice == SliceChoice::kAdmitPending ? running.size()
+                       : choice == SliceChoice::kRunning    ? index
+                                                            : 0;
+  for (size_t k = 0; k < older; ++k)
+    if (running[k].work_left) ++running[k].passes;
+  if (choice == SliceChoice::kRunning) running[index].passes = 0;
 }
 
 }  // namespace muonx

@@ -46,7 +46,7 @@ struct SchedJob {
   uint64_t film_off = 0, film_bytes = 0;
   uint32_t next_sample = 0;  // samples handed to the engine so far
   uint32_t open_slices = 0;  // of those, slices not finished yet
-  uint64_t last_slice_iter = 0;  // engine iteration of its latest slice
+  uint32_t passes = 0;  // slices given to newer work since its last slice (PickSlice)
   int state = 0;  // kPending, kActive, kResolving, kDone
   muonx_job_result result{};
   hipEvent_t ev = nullptr;  // resolve finished
@@ -86,9 +86,9 @@ class Scheduler {
   // slice comes before further slices of running jobs (which continue
   // oldest first), so a small request waits behind about one queue fill,
   // not behind the whole of a large one, while large requests still finish
-  // in arrival order. A running job that got no slice for
-  // kSliceAgeIterations goes first (PickSlice), so no stream of new or
-  // smaller requests starves it. Images do not depend on the slicing.
+  // in arrival order. A running job passed over kSlicePassLimit times by
+  // newer work goes first (PickSlice), so no stream of new or smaller
+  // requests starves it. Images do not depend on the slicing.
   int Step(std::vector<SchedJob> *finished);
   bool Idle() const { return pending_.empty() && jobs_.empty(); }
   size_t pending() const { return pending_.size(); }
@@ -123,7 +123,7 @@ class Scheduler {
   uint64_t snap_count_ = 0;
   uint64_t iterations_ = 0;
   uint64_t progress_iter_ = 0;  // iteration count when a sample last finished
-  unsigned long long rays_[kRayCounters] = {0, 0, 0, 0};
+  unsigned long long rays_[kRayCounters] = {};
 };
 
 uint64_t NowNs();

@@ -157,7 +157,6 @@ int Scheduler::AdmitSlice(SchedJob &j) {
   open_items_ += sl.nitems;
   j.next_sample += n;
   ++j.open_slices;
-  j.last_slice_iter = iterations_;
   return MUONX_OK;
 }
 
@@ -173,7 +172,7 @@ int Scheduler::AdmitWork() {
       if (j.state != kActive) continue;
       const uint32_t spp = uint32_t(j.spec.params.spp);
       const uint64_t left = j.next_sample < spp ? uint64_t(spp - j.next_sample) * j.npix : 0;
-      cands.push_back(SliceCandidate{left, j.last_slice_iter});
+      cands.push_back(SliceCandidate{left, j.passes});
       running.push_back(&j);
     }
     // the oldest waiting job's film, if it fits (admission by film memory,
@@ -181,9 +180,10 @@ int Scheduler::AdmitWork() {
     uint64_t off = 0;
     const bool fits = !pending_.empty() && films_.Alloc(pending_.front().film_bytes, &off);
     size_t idx = 0;
-    const SliceChoice ch =
-        PickSlice(cands, fits, cfg_.slice_order == MUONX_SLICES_LEAST_WORK_FIRST, iterations_, &idx);
+    const SliceChoice ch = PickSlice(cands, fits, cfg_.slice_order == MUONX_SLICES_LEAST_WORK_FIRST, &idx);
     if (ch != SliceChoice::kAdmitPending && fits) films_.Free(off, pending_.front().film_bytes);
+    RecordPick(cands, ch, idx);
+    for (size_t k = 0; k < running.size(); ++k) running[k]->passes = cands[k].passes;
     if (ch == SliceChoice::kAdmitPending) {
       SchedJob &p = pending_.front();
       uint32_t slot = 0;

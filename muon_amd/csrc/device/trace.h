@@ -144,9 +144,11 @@ __device__ __forceinline__ void to_object(const float *xf, V3 o, V3 d, V3 &oo, V
 struct TraceCounters {
   uint32_t nodes = 0;   // internal nodes fetched (two child boxes each)
   uint32_t prims = 0;   // primitive tests (the reference's object tests)
-  uint32_t ohits = 0;   // object hits (exact mode, as muon/acceleration.cc:108-114,177-179)
-  uint32_t btests = 0;  // bounds tests (exact mode, as muon/acceleration.cc:92,162)
-  uint32_t bhits = 0;   // bounds hits (exact mode, as muon/acceleration.cc:101,171)
+  uint32_t ohits = 0;   // object hits (as muon/acceleration.cc:108-114,177-179 count them)
+  // bounds tests / hits: in the exact mode the reference's (muon/acceleration.cc:92,101,162,171);
+  // in the fast mode the child boxes the device tree's nodes test and pass
+  uint32_t btests = 0;
+  uint32_t bhits = 0;
 };
 
 // Tests leaf-order primitive i against the ray; returns the world distance
@@ -301,6 +303,10 @@ __device__ __forceinline__ bool node_step(const DevSceneView &s, const TraceRay 
     const float bm = below(best);
     h0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, bm, lo0);
     h1 = box_hit(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, bm, lo1);
+  }
+  if (COUNT && !EXACT) {  // the fast mode's own work: both child boxes tested
+    cnt.btests += 2;
+    cnt.bhits += uint32_t(h0) + uint32_t(h1);
   }
   const uint32_t axis = (uint32_t(n3.z) >> 29) & 3u;
   const bool swap = (r.dneg >> axis) & 1u;  // near child is children[1]
@@ -502,6 +508,10 @@ __device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay
   float k0 = h0 ? t0 : kInf, k1 = h1 ? t1 : kInf, k2 = h2 ? t2 : kInf, k3 = h3 ? t3 : kInf;
   uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
   const int nhit = int(h0) + int(h1) + int(h2) + int(h3);
+  if (COUNT) {
+    cnt.btests += n;
+    cnt.bhits += uint32_t(nhit);
+  }
   if (nhit == 0) return false;
   auto cswap = [](float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
     const bool sw = kb < ka;
@@ -593,6 +603,10 @@ __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay
   float k0 = h[0] ? t[0] : kInf, k1 = h[1] ? t[1] : kInf, k2 = h[2] ? t[2] : kInf, k3 = h[3] ? t[3] : kInf;
   uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
   const int nhit = int(h[0]) + int(h[1]) + int(h[2]) + int(h[3]);
+  if (COUNT) {
+    cnt.btests += n;
+    cnt.bhits += uint32_t(nhit);
+  }
   if (nhit == 0) return false;
   auto cswap = [](float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
     const bool sw = kb < ka;

@@ -61,6 +61,12 @@ int Renderer::Render() const {
     muonx_host_scene_destroy(hs);
     return Fail("device");
   }
+  if (options_.show_stats) {  // the path tracer counts its tests only when asked (muon's --stats)
+    muonx_tuning t;
+    muonx_ctx_get_tuning(ctx, &t);
+    t.count_work = 1;
+    muonx_ctx_set_tuning(ctx, &t);
+  }
   if (muonx_scene_create(ctx, &desc, &scene) != MUONX_OK) {
     muonx_host_scene_destroy(hs);
     muonx_ctx_destroy(ctx);

@@ -120,6 +120,8 @@ class Renderer:
         hs = HostScene.from_file(self.scene_file, ACCELERATION[o.acceleration], PARTITION[o.partition_strategy],
                                  o.default_seed)
         ctx = Context(o.device)
+        if o.show_stats:  # the path tracer counts its tests only when asked (muon's --stats)
+            ctx.set_tuning(count_work=1)
         ds = DeviceScene(ctx, hs)
         t1 = time.perf_counter()
         accum, stats = render_device(ds, hs.params)

@@ -173,32 +173,44 @@ static void CheckRanges(std::mt19937_64 &rng) {
 
 // Slice choice: new admissions first, then the oldest or least-work render;
 // and under a stream that always has a new render waiting, a running large
-// render still gets a slice at least every kSliceAgeIterations + 1
-// iterations (without the age rule it would get none).
+// render still gets a slice at least every kSlicePassLimit + 1 picks
+// (without the pass count it would get none).
 static void CheckSlicePolicy() {
   size_t idx = 99;
-  std::vector<SliceCandidate> c = {{50, 10}, {0, 10}, {20, 10}};
-  CHECK(PickSlice(c, true, false, 11, &idx) == SliceChoice::kAdmitPending);
-  CHECK(PickSlice(c, false, false, 11, &idx) == SliceChoice::kRunning && idx == 0);
-  CHECK(PickSlice(c, false, true, 11, &idx) == SliceChoice::kRunning && idx == 2);
-  CHECK(PickSlice({{0, 0}}, false, true, 5, &idx) == SliceChoice::kNone);
-  CHECK(PickSlice(c, true, true, 10 + kSliceAgeIterations, &idx) == SliceChoice::kRunning && idx == 0);
+  std::vector<SliceCandidate> c = {{50, 0}, {0, 0}, {20, 0}};
+  CHECK(PickSlice(c, true, false, &idx) == SliceChoice::kAdmitPending);
+  CHECK(PickSlice(c, false, false, &idx) == SliceChoice::kRunning && idx == 0);
+  CHECK(PickSlice(c, false, true, &idx) == SliceChoice::kRunning && idx == 2);
+  CHECK(PickSlice({{0, 0}}, false, true, &idx) == SliceChoice::kNone);
+  // passes: a new admission passes every running render with work left
+  RecordPick(c, SliceChoice::kAdmitPending, 0);
+  CHECK(c[0].passes == 1 && c[1].passes == 0 && c[2].passes == 1);
+  // least-work-first chose render 2: render 0 was passed over, 2 restarts
+  RecordPick(c, SliceChoice::kRunning, 2);
+  CHECK(c[0].passes == 2 && c[2].passes == 0);
+  c[0].passes = kSlicePassLimit;
+  CHECK(PickSlice(c, true, true, &idx) == SliceChoice::kRunning && idx == 0);
   for (int lwf = 0; lwf < 2; ++lwf) {
-    // one large render with 1000 slices of work; each iteration one slice is
-    // handed out and a new one-slice render is always waiting
+    // one large render with 1000 slices of work; at every pick a new
+    // two-slice render is waiting and fits
     std::vector<SliceCandidate> run = {{1000, 0}};
-    uint64_t last = 0, max_gap = 0;
-    for (uint64_t it = 1; it <= 5000; ++it) {
-      const SliceChoice ch = PickSlice(run, true, lwf != 0, it, &idx);
-      if (ch == SliceChoice::kRunning) {
-        CHECK(idx == 0);
-        run[0].work_left -= 1;
-        run[0].last_slice_iter = it;
-        max_gap = std::max(max_gap, it - last);
-        last = it;
+    uint64_t last = 0, max_gap = 0, picks_large = 0;
+    for (uint64_t pick = 1; pick <= 5000; ++pick) {
+      const SliceChoice ch = PickSlice(run, true, lwf != 0, &idx);
+      RecordPick(run, ch, idx);
+      if (ch == SliceChoice::kAdmitPending) {
+        run.push_back({2, 0});
+      } else {
+        CHECK(ch == SliceChoice::kRunning);
+        run[idx].work_left -= 1;
+        if (idx == 0) {
+          ++picks_large;
+          max_gap = std::max(max_gap, pick - last);
+          last = pick;
+        }
       }
     }
-    CHECK(last > 0 && max_gap <= kSliceAgeIterations + 1);
+    CHECK(picks_large > 0 && max_gap <= kSlicePassLimit + 1);
   }
 }
 

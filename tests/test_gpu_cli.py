@@ -96,3 +96,17 @@ def test_cli_flag_errors():
     assert r.returncode == 1 and "unknown flag" in r.stderr
     r = _run(["--scene=x.muon", "--acceleration=octree"])
     assert r.returncode == 1
+
+
+@pytest.mark.gpu
+def test_cli_stats_count_path_tracer_work(tmp_path, ref_dir):
+    """--stats on a path-traced scene prints the traversal's test counters
+    (the device tree's work), not zeros."""
+    import re
+    shutil.copy(os.path.join(ref_dir, "cornell_brdf.muon"), tmp_path / "scene.muon")
+    r = _run(["--scene=%s" % (tmp_path / "scene.muon"), "--output=%s" % (tmp_path / "o.png"), "--stats=true"])
+    assert r.returncode == 0, r.stderr
+    got = {k: int(re.search(k + r"\s*:\s*(\d+)", r.stderr).group(1))
+           for k in ("Primary rays", "Ray-object tests", "Ray-object hits", "Bounds tests", "Bounds hits")}
+    assert got["Primary rays"] > 0 and got["Ray-object tests"] > got["Ray-object hits"] > 0
+    assert got["Bounds tests"] > got["Bounds hits"] > 0

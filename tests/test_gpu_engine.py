@@ -323,8 +323,8 @@ def test_autotune_installs_the_fastest_configuration(resident):
 def test_server_large_job_completes_under_a_request_stream(ctx, resident, order):
     """One large render (128 slices of a 4096-path engine) while a stream
     keeps up to 8 small requests waiting: new requests' first slices go
-    first and least-work-first prefers the small ones, but a render that
-    got no slice for 64 iterations goes ahead (tests/cpu/engine_plan_check.cpp
+    first and least-work-first prefers the small ones, but a render passed
+    over 16 times by newer work goes ahead (tests/cpu/engine_plan_check.cpp
     checks that bound), so the large render finishes while the stream still
     runs, with its image alone."""
     hs, ds = resident["cornell"]
@@ -350,3 +350,47 @@ def test_server_large_job_completes_under_a_request_stream(ctx, resident, order)
             assert srv.wait(t, timeout_ms=60000) is not None
     assert len(small_tickets) >= 16
     assert np.array_equal(done_big[0].cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.fixture
+def counting(ctx):
+    ctx.set_tuning(count_work=1)
+    yield
+    ctx.set_tuning(count_work=0)
+
+
+def test_path_tracer_work_counters_off_by_default(ctx, resident):
+    ds, p, key = job_list(resident)[0]
+    assert ctx.get_tuning()["count_work"] == 0
+    _, _, st = render_batch(ctx, [(ds, p, key)])
+    assert st.primary_rays > 0 and (st.object_tests, st.object_hits, st.bounds_tests, st.bounds_hits) == (0, 0, 0, 0)
+
+
+def test_path_tracer_work_counters(ctx, resident, counting):
+    """With tuning.count_work the path tracer's TraceStats test counters are
+    the device traversal's work (child boxes tested and passed, primitive
+    tests and hits): per ray a pure function of the ray, so a batch's
+    counters equal the sum of its jobs rendered alone, and a plain render's
+    equal its one-job batch's; images do not change."""
+    jobs = job_list(resident)[:3]
+    ctx.set_tuning(count_work=0)
+    ref = alone(jobs[0][0], jobs[0][1])  # not counted
+    ctx.set_tuning(count_work=1)
+    fields = ("primary_rays", "secondary_rays", "shadow_rays", "object_tests", "object_hits", "bounds_tests",
+              "bounds_hits")
+    singles = []
+    for ds, p, key in jobs:
+        _, _, st = render_batch(ctx, [(ds, p, key)])
+        singles.append(st)
+        assert st.object_tests > 0 and 0 < st.object_hits <= st.object_tests
+        assert 0 < st.bounds_hits <= st.bounds_tests
+        assert st.bounds_tests >= st.primary_rays
+    _, _, stb = render_batch(ctx, jobs)
+    for f in fields:
+        assert getattr(stb, f) == sum(getattr(s, f) for s in singles), f
+    ds, p, _ = jobs[0]
+    acc, plain = render_device(ds, p)
+    torch.cuda.synchronize()
+    for f in fields:
+        assert getattr(plain, f) == getattr(singles[0], f), f
+    assert np.array_equal(acc.cpu().numpy().view(np.uint32), ref.view(np.uint32))
