@@ -66,28 +66,8 @@ def log(msg):
 
 
 def build_pool(ctx, log_fn, kinds=None):
-    from muon_amd import DeviceScene, HostScene, scenes
-    pool = []
-    for spec in (kinds or scenes.POOL):
-        kind, size, seed = (tuple(spec) + (0,))[:3]
-        t0 = time.perf_counter()
-        hs = HostScene.from_text(scenes.pool_scene_text(kind, size, seed))
-        ds = DeviceScene(ctx, hs)
-        pool.append((kind, size, hs, ds, ds.info()))
-        log_fn(f"bench: pool scene {kind} {size}: {ds.info()['num_prims']} prims, {time.perf_counter() - t0:.1f} s")
-    return pool
-
-
-def make_job(pool, rng, res_choices, spp_lo, spp_hi, depth=8):
-    from muon_amd import _lib, scenes
-    kind, size, hs, ds, _ = pool[int(rng.integers(len(pool)))]
-    p = _lib.RenderParams.from_buffer_copy(hs.params)
-    r = int(rng.choice(res_choices))
-    scenes.random_pose(p, kind, rng, r, r)
-    p.spp = int(rng.integers(spp_lo, spp_hi + 1))
-    p.max_depth = depth
-    p.seed = int(rng.integers(1 << 31))
-    return ds, p
+    from muon_amd import scenes, workloads
+    return workloads.build_pool(ctx, kinds or scenes.POOL, lambda m: log_fn("bench: " + m))
 
 
 def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 1024), spp_range=(4, 64),
@@ -96,10 +76,10 @@ def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 10
     import torch
     from muon_amd.renderer import render_device
     from muon_amd.serving import engine_config, render_batch
-    rng = np.random.default_rng(seed)
+    from muon_amd.workloads import batched_jobs
     # key 0 for every job: the plain path renders with key 0 too, so both
     # paths must produce the same images (the jobs differ by seed and pose)
-    jobs = [make_job(pool, rng, list(res_choices), spp_range[0], spp_range[1]) + (0,) for _ in range(njobs)]
+    jobs = batched_jobs(pool, njobs, seed, res_choices, spp_range)
     samples = sum(p.width * p.height * p.spp for _, p, _ in jobs)
     film = sum(p.width * p.height * 24 + 256 for _, p, _ in jobs)
     cfg = engine_config(max_jobs=max(64, 2 * njobs), film_budget_bytes=film + (1 << 20), iterations_per_poll=4)
@@ -133,6 +113,125 @@ def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 10
             "job_latency_p99_s": round(float(np.percentile(lat, 99)), 3)}, samples / tb
 
 
+def host_threads():
+    """Host cores this process may run on (all of them for the CPU baseline)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def host_description():
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{model}, nproc {os.cpu_count()}, {host_threads()} usable"
+
+
+def cpu_render_cases():
+    """(key, scene text, sample) of the CPU restatement's render baselines:
+    the reference's CPU-runnable case at full size; the sphere field (10,008
+    spheres, BRDF sampling, NEE off, depth 8) at 512x512 instead of
+    1024x1024; the level-7 icosphere (GGX, NEE + MIS, depth 8) at 128x128
+    instead of 512x512 (the same scenes, spp and estimators on fewer
+    pixels)."""
+    from muon_amd import scenes
+    return [("cornell_256_render", scenes.cornell_text(256, 256, 16, 5, seed=0),
+             "Cornell box 256x256, 16 spp, depth 5 (full size)"),
+            ("sphere_field_render", scenes.sphere_field_text(512, 512, 32, 8),
+             "sphere field 512x512 of the 1024x1024, 32 spp, depth 8"),
+            ("icosphere_render", scenes.icosphere_text(128, 128, 64, 8),
+             "icosphere 128x128 of the 512x512, 64 spp, depth 8")]
+
+
+def read_rates_gbs(dev, num_cus):
+    """Measured read bandwidth of a streaming read kernel
+    (muon_amd/csrc/bench/read_probe.hip, 16-byte loads) over buffers resident
+    in the L2 (16 MiB: 2 MiB per XCD), the MALL (64 MiB) and HBM (8 GiB):
+    {level: GB/s}, the best of 8, 16 and 32 blocks of 256 threads per CU and
+    of 5 timed launches after 3 untimed ones. None if the probe library is
+    missing."""
+    import ctypes
+    import torch
+    path = os.path.join(ROOT, "muon_amd", "lib", "libmuonx_probe.so")
+    if not os.path.exists(path):
+        return None
+    probe = ctypes.CDLL(path)
+    probe.probe_read_launch.restype = ctypes.c_int
+    probe.probe_read_launch.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+    stream = torch.cuda.current_stream(dev)
+    sink = torch.zeros(256, dtype=torch.int32, device=dev)
+    out = {}
+    for level, nbytes, passes in (("l2", 16 << 20, 64), ("mall", 64 << 20, 8), ("hbm", 8 << 30, 1)):
+        buf = torch.zeros(nbytes // 4, dtype=torch.int32, device=dev)
+        best = None
+        for bpc in (8, 16, 32):
+            def launch():
+                return probe.probe_read_launch(buf.data_ptr(), nbytes, passes, bpc * num_cus, sink.data_ptr(),
+                                               stream.cuda_stream)
+            for _ in range(3):
+                assert launch() == 0
+            for _ in range(5):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                assert launch() == 0
+                b.record(stream)
+                torch.cuda.synchronize()
+                ms = a.elapsed_time(b)
+                best = ms if best is None else min(best, ms)
+        out[level] = round(nbytes * passes / (best / 1e3) / 1e9, 1)
+        del buf
+    return out
+
+
+def roofline_of(rays, mean_ms, dev_nodes, dev_prims, ref_nodes, ref_objs, scene_bytes, dev):
+    """Roofline of the closest-hit kernel (see DESIGN.md for the byte models).
+
+    achieved: the device tree's algorithmic bytes per ray, 24 (origin,
+    direction) + 8 (prim id, distance) + 64 per quantized 4-wide node fetched
+    + 48 per triangle test, times the rays of one launch, over the mean
+    launch time; frac against the 8 TB/s HBM peak. The scene is resident in
+    the 256 MB MALL, so the same bytes are also set against the measured
+    read rate of a MALL-resident buffer (mall_frac; read_measured_gbs holds
+    the probe's L2, MALL and HBM rates) and the kernel's
+    measured traffic past L2 (rocprofv3 FETCH_SIZE / WRITE_SIZE, from
+    profiles/traffic_closest_icosphere.json) is reported as `traffic`.
+    reference_equivalent: the bytes the reference's binary SAH tree would move
+    for the same rays, 32 + 16 + 32 per bounds test + 36 per object test
+    (its counters measured by the oracle on a sample of the rays)."""
+    b_dev = 24 + 8 + 64 * dev_nodes + 48 * dev_prims
+    achieved = b_dev * rays / (mean_ms / 1e3) / 1e9
+    b_ref = 32 + 16 + 32 * ref_nodes + 36 * ref_objs
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_closest_icosphere.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    import torch
+    rates = read_rates_gbs(dev, torch.cuda.get_device_properties(dev).multi_processor_count) or {}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "bytes_per_ray": round(b_dev, 1), "device_nodes_per_ray": round(dev_nodes, 2),
+           "device_prim_tests_per_ray": round(dev_prims, 2), "scene_bytes": int(scene_bytes),
+           "read_measured_gbs": rates or None,
+           "mall_frac": round(achieved / rates["mall"], 4) if rates else None,
+           "reference_equivalent": {"bytes_per_ray": round(b_ref, 1), "gbs": round(b_ref * rays / (mean_ms / 1e3) / 1e9, 1),
+                                    "bounds_tests_per_ray": round(ref_nodes, 2),
+                                    "object_tests_per_ray": round(ref_objs, 2)}}
+    if traffic:
+        out["traffic_gbs"] = round(traffic / (mean_ms / 1e3) / 1e9, 1)
+        out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    return out
+
+
 def relmse(x, r):
     import numpy as np
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
@@ -142,11 +241,11 @@ def bench_serving(ctx, pool, nreq, capacity_samples_s, load, seed, log_fn):
     import numpy as np
     import torch
     from muon_amd.serving import Server, engine_config, render_batch
-    rng = np.random.default_rng(seed + 1)
-    reqs = [make_job(pool, rng, [256, 512, 1024], 8, 128) for _ in range(nreq)]
+    from muon_amd.workloads import poisson_gaps, served_requests
+    reqs = served_requests(pool, nreq, seed)
     mean_samples = float(np.mean([p.width * p.height * p.spp for _, p in reqs]))
     rate = load * capacity_samples_s / mean_samples
-    gaps = rng.exponential(1.0 / rate, nreq)
+    gaps = poisson_gaps(nreq, rate, seed)
     cfg = engine_config(max_jobs=256, film_budget_bytes=1 << 30, iterations_per_poll=2)
     log_fn(f"bench: serving {nreq} requests at {rate:.2f}/s (load {load})")
     outs = {}
@@ -282,28 +381,13 @@ def main():
     import _oracle
     orc = _oracle.Oracle()
     oscene = orc.load_text(text)
+    cpu_threads = host_threads()
     sample = 1 << 16
-    _, _, _, _, st = oscene.intersect(o_np[:sample], d_np[:sample], threads=min(16, os.cpu_count() or 8))
+    _, _, _, _, st = oscene.intersect(o_np[:sample], d_np[:sample], threads=cpu_threads)
     n_node = st[4] / sample
     n_obj = st[2] / sample
-    b_ray = 32 + 16 + 32 * n_node + 36 * n_obj
-    achieved = b_ray * args.rays / (mean_ms / 1e3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_closest_icosphere.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                # the bytes the kernel actually moves past L2 (profiled), per second of this run
-                "traffic_gbs": round(traffic / (mean_ms / 1e3) / 1e9, 1) if traffic else None,
-                "traffic_frac": round(traffic / (mean_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-                "bytes_per_ray": round(b_ray, 1), "ref_bounds_tests_per_ray": round(n_node, 2),
-                "ref_object_tests_per_ray": round(n_obj, 2),
-                "device_nodes_per_ray": round(dev_nodes / args.rays, 2),
-                "device_prim_tests_per_ray": round(dev_prims / args.rays, 2)}
+    roofline = roofline_of(args.rays, mean_ms, dev_nodes / args.rays, dev_prims / args.rays, n_node, n_obj,
+                           info["device_bytes"], dev)
 
     # the autotuner picks the launch configuration for this scene and the C1
     # render job; the render, batched and serving measurements below use it
@@ -319,7 +403,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 8)
+        threads = cpu_threads
         done, secs, chunk = 0, 0.0, 1 << 18
         while secs < args.cpu_seconds and done < args.rays:
             n = min(chunk, args.rays - done)
@@ -328,17 +412,24 @@ def main():
             secs += time.perf_counter() - t0
             done += n
         cpu = {"value": round(done / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"first {done} of the same microbench rays, closest hit, {threads} threads, {secs:.1f} s"}
-        log(f"bench: cpu baseline {cpu['value']} Mrays/s")
+               "sample": f"first {done} of the same microbench rays, closest hit, {threads} threads, {secs:.1f} s",
+               "host": host_description()}
+        log(f"bench: cpu baseline {cpu['value']} Mrays/s on {threads} threads ({cpu['host']})")
         if not args.no_render:
-            # the reference's own CPU case: the Cornell box at 256x256, 16 spp,
-            # depth 5, rendered by the CPU restatement (render time as the
-            # reference's Stats measures it: tiles only, no parse / build)
-            c0 = orc.load_text(scenes.cornell_text(256, 256, 16, 5, seed=0))
-            _, _, cst, csecs = c0.render(threads)
-            cpu["cornell_256_render"] = {"seconds": round(float(csecs), 3), "threads": threads,
-                                         "reference_rays": int(cst[0] + cst[1])}
-            log(f"bench: cpu cornell 256x256 16spp render {csecs:.2f} s")
+            # the CPU restatement rendering the configurations (render time as
+            # the reference's Stats measures it: tiles only, no parse / build):
+            # the reference's own CPU case at full size, the sphere field and
+            # the icosphere at reduced films (bounded samples of the same
+            # scenes, estimator and depth)
+            for key, scene_text, label in cpu_render_cases():
+                c = orc.load_text(scene_text)
+                _, _, cst, csecs = c.render(threads)
+                rays = int(cst[0] + cst[1] + cst[6])
+                cpu[key] = {"seconds": round(float(csecs), 3), "threads": threads, "sample": label,
+                            "reference_rays": int(cst[0] + cst[1]), "rays": rays,
+                            "mrays_per_s": round(rays / csecs / 1e6, 3)}
+                log(f"bench: cpu {key}: {label}: {csecs:.2f} s, {cpu[key]['mrays_per_s']} Mrays/s")
+                del c
 
     def timed_render(scene, params):
         accum, rst = render_device(scene, params)  # warm the workspace
@@ -380,6 +471,13 @@ def main():
         if cpu and "cornell_256_render" in cpu:
             render["cornell_256"]["gpu_over_cpu"] = round(cpu["cornell_256_render"]["seconds"] /
                                                           render["cornell_256"]["seconds"], 1)
+        # Mrays/s ratios of the full-size GPU renders over the CPU samples of
+        # the same scenes
+        if cpu:
+            for rkey, ckey in (("icosphere", "icosphere_render"), ("sphere_field", "sphere_field_render")):
+                g = render if rkey == "icosphere" else render.get(rkey)
+                if g and ckey in cpu and "mrays_per_s" in g:
+                    cpu[ckey]["gpu_over_cpu_mrays"] = round(g["mrays_per_s"] / cpu[ckey]["mrays_per_s"], 1)
         log(f"bench: cornell 256 {render['cornell_256']}")
         del ds0
 
