@@ -68,6 +68,9 @@ typedef struct muonx_options {
   int32_t parallelism;        /* host threads for CPU-side work */
   int32_t show_stats;
   int64_t default_seed;       /* >= 0 replaces the std::random_device seed (muon/parser.cc:181) */
+  int32_t device_bvh;         /* 1 (with acceleration = bvh): skip the host SAH build; muonx_scene_create
+                                 then builds a BVH on the device (fast admission, fast traversal mode
+                                 only). 0 (default): the reference's SAH tree is built on the host */
 } muonx_options;
 
 /* Scene descriptor handed to the device: host arrays in the reference's data
@@ -106,6 +109,14 @@ typedef struct muonx_scene_desc {
                                       one leaf in parse order, whose queries count tests as
                                       Linear does */
 } muonx_scene_desc;
+/* A descriptor with num_nodes = 0 (the node arrays and leaf_prims unused)
+ * hands over primitives without a tree: muonx_scene_create then builds a
+ * binary BVH on the device (a linear BVH over Morton codes of the primitive
+ * centroids) and uses it as it uses a host-built one. Such a scene is not
+ * traversed in the reference's node order, so the exact traversal mode and
+ * the integrators that need it (normals, albedo, depth, raytracer,
+ * analyticdirect) are refused with MUONX_ERR_STATE; the path tracer and fast
+ * queries run as on a host-built tree. */
 
 /* Per-render settings: the scene's render properties (muon/scene.h:31-51)
  * and the pinhole camera precomputed on the host (muon/camera.cc:15-35). */
@@ -159,6 +170,8 @@ typedef struct muonx_scene_info {
   uint32_t bvh_depth;     /* binary SAH tree depth */
   uint32_t num_lights;
   uint32_t num_quad_lights;
+  uint32_t device_built;  /* 1: the BVH was built on the device (descriptor without a tree) */
+  float build_ms;         /* that build: kernels, sort, copies and the host's pre-order pass */
 } muonx_scene_info;
 
 /* ------------------------------------------------------------ misc */
@@ -235,8 +248,9 @@ int muonx_autotune(muonx_scene *scene, const muonx_render_params *job, uint32_t 
                    muonx_autotune_report *report);
 
 /* Uploads a scene; replaces the Structure::Init()'d acceleration structure
- * (muon/acceleration.h:26-53) with a device-resident BVH. Immutable after
- * creation and shareable by concurrent renders. */
+ * (muon/acceleration.h:26-53) with a device-resident BVH (the descriptor's
+ * host-built tree, or one built on the device when it has none). Immutable
+ * after creation and shareable by concurrent renders. */
 int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *desc, muonx_scene **out);
 /* Waits for server jobs that still render the scene, then frees it. */
 void muonx_scene_destroy(muonx_scene *scene);

@@ -14,7 +14,7 @@ class MuonxError(RuntimeError):
 
 class Options(C.Structure):
     _fields_ = [("acceleration", C.c_int32), ("partition_strategy", C.c_int32), ("parallelism", C.c_int32),
-                ("show_stats", C.c_int32), ("default_seed", C.c_int64)]
+                ("show_stats", C.c_int32), ("default_seed", C.c_int64), ("device_bvh", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -69,7 +69,8 @@ class AutotuneReport(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("device_bytes", C.c_uint64), ("num_prims", C.c_uint32), ("num_nodes", C.c_uint32),
-                ("bvh_depth", C.c_uint32), ("num_lights", C.c_uint32), ("num_quad_lights", C.c_uint32)]
+                ("bvh_depth", C.c_uint32), ("num_lights", C.c_uint32), ("num_quad_lights", C.c_uint32),
+                ("device_built", C.c_uint32), ("build_ms", C.c_float)]
 
 
 class EngineConfig(C.Structure):

@@ -96,12 +96,13 @@ int muonx_ctx_get_tuning(const muonx_ctx *ctx, muonx_tuning *t) {
 namespace {
 int ParseInto(const std::string &text, const std::string &base_dir, const muonx_options *opts,
               muonx_host_scene **out) {
-  muonx_options o{1, 2, 1, 1, -1};
+  muonx_options o{1, 2, 1, 1, -1, 0};
   if (opts) o = *opts;
   auto *hs = new (std::nothrow) muonx_host_scene();
   if (!hs) return MUONX_ERR_NOMEM;
   try {
-    host::ParseScene(text, o.acceleration, o.partition_strategy, o.default_seed, hs->sc, base_dir);
+    host::ParseScene(text, o.acceleration, o.partition_strategy, o.default_seed, hs->sc, base_dir,
+                     o.device_bvh == 0);
   } catch (const std::exception &e) {
     delete hs;
     SetError(std::string("parse failed: ") + e.what());

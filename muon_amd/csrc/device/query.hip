@@ -294,6 +294,10 @@ int CheckQuery(const char *fn, muonx_scene *scene, const muonx_query *q) {
 }
 
 int RunQuery(muonx_scene *scene, const muonx_query *q, void *stream, bool noalloc) {
+  if (q->mode == MUONX_TRAVERSAL_EXACT && !scene->reference_tree) {
+    SetError("exact traversal needs the reference's SAH tree; this scene's BVH was built on the device");
+    return MUONX_ERR_STATE;
+  }
   std::lock_guard<std::mutex> lock(scene->ctx->mu);
   MX_HIP(hipSetDevice(scene->ctx->device));
   return LaunchQuery(scene->ctx, scene->view, q->any_hit != 0, q->mode == MUONX_TRAVERSAL_EXACT, q->ray_o, q->ray_d,

@@ -295,6 +295,11 @@ extern "C" int muonx_render(muonx_scene *scene, const muonx_render_params *param
     SetError("muonx_render: scene has no camera");  // the reference dereferences a null camera
     return MUONX_ERR_INVALID;
   }
+  if (p.integrator != MUONX_INTEGRATOR_PATHTRACER && !scene->reference_tree) {
+    SetError("muonx_render: this integrator traverses the reference's SAH tree; the scene's BVH was built on "
+             "the device");
+    return MUONX_ERR_STATE;
+  }
   muonx_ctx *ctx = scene->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
   MX_HIP(hipSetDevice(ctx->device));

@@ -87,6 +87,8 @@ struct muonx_scene {
   muonx::DevSceneView view{};
   uint32_t bvh_depth = 0;
   uint64_t device_bytes = 0;
+  bool reference_tree = true;  // false: the BVH was built on the device (exact traversal refused)
+  float build_ms = 0.0f;       // that device build
   float bounds[6] = {0, 0, 0, 0, 0, 0};  // world bounds: min xyz, max xyz
   std::vector<float> host_lights;     // 16 per light
   std::vector<float> host_materials;  // 16 per material

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../common/vecmath.h"
+#include "bvh_device.h"
 #include "runtime.h"
 
 namespace muonx {
@@ -275,25 +276,53 @@ bool SceneLive(uint64_t id) {
 
 using namespace muonx;
 
-extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muonx_scene **out) {
-  if (!ctx || !d || !out) {
+extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *din, muonx_scene **out) {
+  if (!ctx || !din || !out) {
     SetError("muonx_scene_create: null argument");
     return MUONX_ERR_INVALID;
   }
   *out = nullptr;
-  const uint32_t np = d->num_prims;
-  if (np == 0 || d->num_nodes == 0) {
+  const uint32_t np = din->num_prims;
+  if (np == 0) {
     SetError("muonx_scene_create: scene has no primitives");  // the reference dereferences a null BVH root
     return MUONX_ERR_INVALID;
   }
-  if (!d->prim_type || !d->prim_geom || !d->prim_transform || !d->prim_material || !d->node_bounds ||
-      !d->node_left || !d->node_right || !d->node_axis || !d->node_start || !d->node_count || !d->leaf_prims ||
-      !d->transforms || !d->materials) {
+  if (!din->prim_type || !din->prim_geom || !din->prim_transform || !din->prim_material || !din->transforms ||
+      !din->materials) {
     SetError("muonx_scene_create: descriptor is missing arrays");
     return MUONX_ERR_INVALID;
   }
+  for (uint32_t i = 0; i < np; ++i) {
+    if (din->prim_transform[i] < 0 || uint32_t(din->prim_transform[i]) >= din->num_transforms) {
+      SetError("muonx_scene_create: invalid transform index");
+      return MUONX_ERR_INVALID;
+    }
+  }
   std::lock_guard<std::mutex> lock(ctx->mu);
   MX_HIP(hipSetDevice(ctx->device));
+  // a descriptor without a tree: build one on the device
+  DeviceBvh built;
+  muonx_scene_desc dd = *din;
+  const bool device_tree = din->num_nodes == 0;
+  if (device_tree) {
+    const int brc = BuildDeviceBvh(ctx, din, &built);
+    if (brc != MUONX_OK) return brc;
+    dd.num_nodes = uint32_t(built.left.size());
+    dd.node_bounds = built.bounds.data();
+    dd.node_left = built.left.data();
+    dd.node_right = built.right.data();
+    dd.node_axis = built.axis.data();
+    dd.node_start = built.start.data();
+    dd.node_count = built.count.data();
+    dd.leaf_prims = built.leaf.data();
+    dd.acceleration = 1;
+  }
+  const muonx_scene_desc *d = &dd;
+  if (!d->node_bounds || !d->node_left || !d->node_right || !d->node_axis || !d->node_start || !d->node_count ||
+      !d->leaf_prims) {
+    SetError("muonx_scene_create: descriptor is missing arrays");
+    return MUONX_ERR_INVALID;
+  }
 
   // ---- primitives in leaf order (+ one sentinel that never hits)
   std::vector<float4> prims(size_t(np + 1) * 3, make_float4(0, 0, 0, 0));
@@ -511,6 +540,8 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *d, muo
   v.bvh_depth = depth;
   s->bvh_depth = depth;
   s->device_bytes = total;
+  s->reference_tree = !device_tree;
+  s->build_ms = device_tree ? built.total_ms : 0.0f;
   std::memcpy(s->bounds, d->node_bounds, sizeof(s->bounds));  // the root's box
   s->host_lights = std::move(lights);
   s->host_materials = std::move(mats);
@@ -539,5 +570,7 @@ extern "C" int muonx_scene_get_info(const muonx_scene *s, muonx_scene_info *out)
   out->bvh_depth = s->bvh_depth;
   out->num_lights = s->view.num_lights;
   out->num_quad_lights = s->view.num_quad_lights;
+  out->device_built = s->reference_tree ? 0u : 1u;
+  out->build_ms = s->build_ms;
   return MUONX_OK;
 }

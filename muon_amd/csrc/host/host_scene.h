@@ -60,8 +60,10 @@ struct HostScene {
 // partition: 0 uniform, 1 midpoint, 2 SAH; acceleration 0 = linear (one leaf).
 // `load` paths are taken relative to base_dir (the scene file's directory;
 // empty: the working directory).
+// With build_bvh false (and the BVH structure) the tree is left empty for
+// the device to build.
 void ParseScene(const std::string &text, int acceleration, int partition, int64_t default_seed,
-                HostScene &out, const std::string &base_dir = std::string());
+                HostScene &out, const std::string &base_dir = std::string(), bool build_bvh = true);
 
 // Builds the binary BVH of muon/acceleration.cc:204-458 into `sc`.
 void BuildBVH(HostScene &sc, int acceleration, int partition);

@@ -87,7 +87,7 @@ bool ParseOnOff(std::istringstream &iss, bool &out) {
 }  // namespace
 
 void ParseScene(const std::string &text, int acceleration, int partition, int64_t default_seed,
-                HostScene &sc, const std::string &base_dir) {
+                HostScene &sc, const std::string &base_dir, bool build_bvh) {
   std::mt19937 seedgen(default_seed >= 0 ? static_cast<uint32_t>(default_seed) : std::random_device()());
   MaterialState mat;
   std::vector<M4> stack{Identity()};
@@ -457,7 +457,11 @@ void ParseScene(const std::string &text, int acceleration, int partition, int64_
       sc.prim_vnormal[t.prim * 9 + 3 * k + 2] = v.normal.z;
     }
   }
-  BuildBVH(sc, acceleration, partition);
+  if (build_bvh || acceleration == 0) {
+    BuildBVH(sc, acceleration, partition);
+  } else {
+    sc.acceleration = 1;  // the BVH is built on the device at upload
+  }
 }
 
 muonx_scene_desc HostScene::Desc() const {

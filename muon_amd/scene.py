@@ -27,15 +27,17 @@ class HostScene:
         check(lib.muonx_host_scene_params(self.h, C.byref(self.params)))
 
     @classmethod
-    def from_file(cls, path, acceleration=1, partition_strategy=2, default_seed=-1):
-        opts = _lib.Options(acceleration, partition_strategy, 1, 1, default_seed)
+    def from_file(cls, path, acceleration=1, partition_strategy=2, default_seed=-1, device_bvh=False):
+        """device_bvh: skip the host SAH build; the device builds a BVH when
+        the scene is uploaded (fast admission; fast traversal mode only)."""
+        opts = _lib.Options(acceleration, partition_strategy, 1, 1, default_seed, int(device_bvh))
         out = C.c_void_p()
         check(lib.muonx_host_scene_parse_file(path.encode(), C.byref(opts), C.byref(out)))
         return cls(out.value)
 
     @classmethod
-    def from_text(cls, text, acceleration=1, partition_strategy=2, default_seed=-1):
-        opts = _lib.Options(acceleration, partition_strategy, 1, 1, default_seed)
+    def from_text(cls, text, acceleration=1, partition_strategy=2, default_seed=-1, device_bvh=False):
+        opts = _lib.Options(acceleration, partition_strategy, 1, 1, default_seed, int(device_bvh))
         out = C.c_void_p()
         b = text.encode()
         check(lib.muonx_host_scene_parse_text(b, len(b), C.byref(opts), C.byref(out)))

@@ -144,3 +144,21 @@ def test_scene_output_name():
     assert HostScene.from_text("output first.png\noutput second.png\n").output == "second.png"
     hs = HostScene.from_text("output keep.png\noutput\n")
     assert hs.output == "keep.png" and hs.warnings == 1
+
+
+def test_device_bvh_parse_skips_the_host_tree(ref_dir):
+    """device_bvh=True parses the same scene (primitives, transforms,
+    materials, lights, render settings) with no host tree; the linear
+    structure is still built on the host (one leaf)."""
+    import os
+    path = os.path.join(ref_dir, "cornell_phong.muon")
+    a = HostScene.from_file(path, default_seed=1)
+    b = HostScene.from_file(path, default_seed=1, device_bvh=True)
+    assert b.desc.num_nodes == 0 and b.desc.num_prims == a.desc.num_prims > 0
+    aa, bb = a.arrays(), b.arrays()
+    for k in ("prim_type", "prim_geom", "prim_transform", "prim_material", "prim_light", "transforms", "materials",
+              "lights"):
+        assert np.array_equal(aa[k], bb[k]), k
+    assert bytes(a.params) == bytes(b.params)
+    c = HostScene.from_file(path, acceleration=0, device_bvh=True)
+    assert c.desc.num_nodes == 1
