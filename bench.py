@@ -7,19 +7,27 @@ icosphere (327,680 tris) inside the Cornell enclosure (36 more tris). Rays
 and the scene are resident in HBM before the timed region. value = rays /
 second over K timed launches (HIP events on the launch stream).
 
-Roofline (HBM-bound path, no dense contraction): algorithmic bytes per ray
-  B_ray = 32 (ray in) + 16 (hit out) + 32 * N_node + 36 * N_tri
-where N_node / N_tri are the reference BVH's bounds / object tests per ray
-(muon/acceleration.cc:92,108), counted by the CPU restatement on a sample of
-the same rays; achieved = B_ray * rays / mean launch time, peak 8 TB/s. The
-4-wide quantized tree touches far fewer bytes than the reference's binary
-tree, so this fraction can exceed 1; traffic_gbs / traffic_frac give the
-bytes the kernel actually moves past L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE
-per launch, read from the committed profile) over the same launch time.
+Roofline (no dense contraction; memory-bound family): `achieved` is the
+device tree's algorithmic bytes per ray,
+  24 (ray in) + 8 (hit out) + 64 * N_node + 48 * N_tri,
+with N_node / N_tri the quantized 4-wide nodes fetched and triangles tested
+per ray (the traversal's own counters on the same rays), times the rays of
+one launch over the mean launch time, against the 8 TB/s HBM peak (frac).
+The scene (58 MB) is resident in the 256 MB MALL, so the same bytes are also
+set against the MALL read rate measured in the run by a streaming-read
+probe (mall_frac; read_measured_gbs holds its L2, MALL and HBM rates).
+`traffic` is the kernel's measured traffic past L2 per launch (rocprofv3
+FETCH_SIZE / WRITE_SIZE scaled by a copy of known size, from
+profiles/traffic_closest_icosphere.json). reference_equivalent gives the
+bytes the reference's binary SAH tree would move for the same rays,
+32 + 16 + 32 per bounds test + 36 per object test (its counters measured by
+the CPU restatement on a sample of the rays).
 
 CPU baseline: the CPU restatement of the reference (same BVH, same
-per-primitive object-space tests) on a bounded sample of the same rays,
-timed on this host.
+per-primitive object-space tests) on all host cores: a bounded sample of the
+same rays, and renders of the Cornell box at full size, the sphere field at
+512x512 and the icosphere at 128x128 (same spp, estimators and depth as the
+GPU renders), each with its GPU/CPU ratio.
 
 Also reported (outside the timed region), after muonx_autotune has chosen the
 launch configuration for this scene and render (the microbench above runs
@@ -28,25 +36,27 @@ with the library's default configuration):
   depth 8, BRDF IS + NEE + MIS); render.sphere_field: the 10,000-sphere field
   with 8 emissive spheres (1024x1024, 32 spp, depth 8, BRDF sampling, NEE off);
   render.cornell_256: the Cornell box at 256x256, 16 spp, depth 5 (the
-  reference's CPU-runnable case), with the CPU restatement's render time of
-  the same image in cpu_baseline.cornell_256_render.
-* batched: 64 render jobs of uneven size (resolution uniform in
-  {128^2, 256^2, 512^2, 1024^2}, spp uniform in [4, 64], depth 8, seeded
-  camera poses)
-  over the resident scene pool (icospheres of 1.3k..1.3M tris, sphere
-  fields), rendered as one batch on the persistent engine and as 64
-  sequential plain muonx_render calls; images/s of both and whether every
-  image is bit-identical between the two.
-  The same comparison for 128 small jobs (128^2, spp 4..16), where per-job
-  launch, tail and synchronisation costs dominate, is reported as
-  batched.small_jobs.
+  reference's CPU-runnable case).
+* admission: the level-8 icosphere (1.31M tris) made resident with the
+  host's SAH build and with the BVH built on the device, and the
+  closest-hit rate of each tree.
+* batched: 64 jobs, each on its own scene (a displaced icosphere or a
+  sphere field, primitives log-uniform in [1k, 2M], about 17.6M in all,
+  BVHs built on the device), resolution uniform in {128^2 .. 1024^2}, spp
+  uniform in [4, 64], depth 8, seeded poses; rendered as one batch on the
+  persistent engine and as 64 sequential plain muonx_render calls: images/s
+  of both, whether every image is bit-identical between the two (so the
+  per-image relMSE is equal), and the relMSE of up to 6 of the images
+  against converged renders (16,384 spp). The same for 128 small jobs
+  (128^2, spp 4..16) as batched.small_jobs; batched.converged_cross_check
+  sets the GPU's converged reference against the CPU restatement at 1,024 spp.
 * serving: the request loop under a Poisson stream of requests (resolution
   {256^2, 512^2, 1024^2}, spp 8..128, random scene and pose) over its own 8
   resident scenes (icospheres of 20k..1.3M tris, four of them at 1.3M, and
   a sphere field: about 5.7M primitives) at a fixed rate chosen as a load
   fraction of the batched throughput of the same request mix on those
-  scenes; images/s, p50/p99 latency (submit to film written) and relMSE of
-  a few requests against a high-spp render of the same request.
+  scenes; images/s, p50/p99 latency (submit to film written) and the relMSE
+  of the four smallest requests against converged renders.
 """
 import argparse
 import json
@@ -71,7 +81,7 @@ def build_pool(ctx, log_fn, kinds=None):
 
 
 def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 1024), spp_range=(4, 64),
-                  sequential=True):
+                  sequential=True, jobs=None):
     import numpy as np
     import torch
     from muon_amd.renderer import render_device
@@ -79,7 +89,9 @@ def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 10
     from muon_amd.workloads import batched_jobs
     # key 0 for every job: the plain path renders with key 0 too, so both
     # paths must produce the same images (the jobs differ by seed and pose)
-    jobs = batched_jobs(pool, njobs, seed, res_choices, spp_range)
+    if jobs is None:
+        jobs = batched_jobs(pool, njobs, seed, res_choices, spp_range)
+    njobs = len(jobs)
     samples = sum(p.width * p.height * p.spp for _, p, _ in jobs)
     film = sum(p.width * p.height * 24 + 256 for _, p, _ in jobs)
     cfg = engine_config(max_jobs=max(64, 2 * njobs), film_budget_bytes=film + (1 << 20), iterations_per_poll=4)
@@ -104,13 +116,19 @@ def bench_batched(ctx, pool, njobs, seed, log_fn, res_choices=(128, 256, 512, 10
     torch.cuda.synchronize()
     ts = time.perf_counter() - t0
     same = sum(int(torch.equal(a, b)) for a, b in zip(accs, seq))
+    # per-image relMSE against converged renders, for up to 6 of the jobs at
+    # 128^2 or 256^2 (equal for the batch and the plain renders: the images
+    # are identical)
+    small = [i for i, (_, p, _) in enumerate(jobs) if p.width * p.height <= 256 * 256][:6]
+    quality = relmse_vs_converged(ctx, [(jobs[i][0], jobs[i][1], accs[i]) for i in small], 5000)
     lat = [r.latency_s for r in results]
     return {"jobs": njobs, "samples": samples, "batched_s": round(tb, 3), "sequential_s": round(ts, 3),
             "images_per_s": round(njobs / tb, 3), "sequential_images_per_s": round(njobs / ts, 3),
             "speedup": round(ts / tb, 3), "mrays_per_s": round(rays / tb / 1e6, 1),
             "msamples_per_s": round(samples / tb / 1e6, 1), "identical_to_plain": f"{same}/{njobs}",
             "job_latency_p50_s": round(float(np.percentile(lat, 50)), 3),
-            "job_latency_p99_s": round(float(np.percentile(lat, 99)), 3)}, samples / tb
+            "job_latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+            "relmse_vs_converged": quality}, samples / tb
 
 
 def host_threads():
@@ -232,6 +250,106 @@ def roofline_of(rays, mean_ms, dev_nodes, dev_prims, ref_nodes, ref_objs, scene_
     return out
 
 
+CONVERGED_SPP = 16384
+
+
+def relmse_vs_converged(ctx, items, key_base):
+    """relMSE (SURVEY's definition: mean over pixels and channels of
+    (x - r)^2 / (r^2 + 1e-2) on the linear film / spp) of each rendered image
+    against a converged render of the same job at 16,384 spp with another
+    random stream. items: (DeviceScene, RenderParams, accum tensor). Returns
+    a list of {"res", "spp", "relmse", "spp_for_1e-3"} (the spp at which the
+    relMSE, falling as 1/spp, would reach 1e-3)."""
+    import numpy as np
+    from muon_amd import _lib
+    from muon_amd.serving import render_batch
+    out = []
+    for k, (ds, p, acc) in enumerate(items):
+        q = _lib.RenderParams.from_buffer_copy(p)
+        q.spp = CONVERGED_SPP
+        (ref,), _, _ = render_batch(ctx, [(ds, q, key_base + k)])
+        rel = relmse(acc.cpu().numpy() / p.spp, ref.cpu().numpy() / CONVERGED_SPP)
+        out.append({"res": f"{p.width}x{p.height}", "spp": int(p.spp), "relmse": round(rel, 6),
+                    "spp_for_1e-3": int(np.ceil(p.spp * rel / 1e-3))})
+        del ref
+    return out
+
+
+def converged_cross_check(ctx, orc, threads, log_fn):
+    """The converged GPU reference against the oracle: the Cornell box at
+    64x64 (BRDF sampling + NEE + MIS, depth 5) rendered at 16,384 spp on the
+    GPU, and at 1,024 spp by the oracle and by the GPU; the two 1,024-spp
+    images must sit at the same relMSE from the GPU reference (the oracle's
+    own estimator and random streams against the GPU's)."""
+    from muon_amd import DeviceScene, HostScene, scenes
+    from muon_amd.serving import render_batch
+    from muon_amd import _lib
+    text = scenes.cornell_text(64, 64, 1024, 5, seed=0)
+    hs = HostScene.from_text(text)
+    ds = DeviceScene(ctx, hs)
+    q = _lib.RenderParams.from_buffer_copy(hs.params)
+    q.spp = CONVERGED_SPP
+    (ref,), _, _ = render_batch(ctx, [(ds, q, 77)])
+    (g1k,), _, _ = render_batch(ctx, [(ds, hs.params, 78)])
+    r = ref.cpu().numpy() / CONVERGED_SPP
+    acc, _, _, secs = orc.load_text(text).render(threads)
+    out = {"scene": "Cornell box 64x64, depth 5", "reference_spp": CONVERGED_SPP, "spp": 1024,
+           "gpu_relmse": round(relmse(g1k.cpu().numpy() / 1024, r), 6),
+           "oracle_relmse": round(relmse(acc / 1024, r), 6), "oracle_seconds": round(float(secs), 2)}
+    out["ratio"] = round(out["gpu_relmse"] / out["oracle_relmse"], 3)
+    log_fn(f"bench: converged-reference cross-check {out}")
+    return out
+
+
+def bench_admission(ctx, log_fn):
+    """Time to make a scene resident: the displaced level-8 icosphere
+    (1,310,732 primitives) ingested with the host's SAH build (the
+    reference's tree, as .muon text) and loaded from binary PLY with its BVH
+    built on the device; the closest-hit rate of the two trees on 2^22
+    incoherent rays."""
+    import tempfile
+    import torch
+    from muon_amd import DeviceScene, HostScene, scenes
+    out = {"scene": "displaced icosphere, level 8"}
+    text = scenes.icosphere_text(256, 256, 16, 8, level=8, seed=0)
+    t0 = time.perf_counter()
+    hs = HostScene.from_text(text)
+    t1 = time.perf_counter()
+    ds = DeviceScene(ctx, hs)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["host_sah"] = {"parse_and_build_s": round(t1 - t0, 3), "upload_s": round(t2 - t1, 3),
+                       "total_s": round(t2 - t0, 3), "prims": ds.info()["num_prims"]}
+    with tempfile.TemporaryDirectory() as tmp:
+        ptext = scenes.icosphere_text(256, 256, 16, 8, level=8, seed=0, mesh_file=f"{tmp}/ico8.ply")
+        t0 = time.perf_counter()
+        hd = HostScene.from_text(ptext, device_bvh=True)
+        t1 = time.perf_counter()
+        dd = DeviceScene(ctx, hd)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+    info = dd.info()
+    out["device_bvh"] = {"parse_s": round(t1 - t0, 3), "build_and_upload_s": round(t2 - t1, 3),
+                         "build_ms": round(info["build_ms"], 1), "total_s": round(t2 - t0, 3),
+                         "prims": info["num_prims"]}
+    lo, hi = scenes.icosphere_world_bounds(level=8)
+    o_np, d_np = scenes.microbench_rays(1 << 22, lo, hi, seed=1)
+    dev = torch.device("cuda:0")
+    o, d = torch.from_numpy(o_np).to(dev), torch.from_numpy(d_np).to(dev)
+    for key, scene in (("host_sah", ds), ("device_bvh", dd)):
+        for _ in range(2):
+            scene.intersect_closest(o, d)
+        stream = torch.cuda.current_stream(dev)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(5):
+            scene.intersect_closest(o, d)
+        b.record(stream)
+        torch.cuda.synchronize()
+        out[key]["closest_hit_mrays_per_s"] = round(5 * (1 << 22) / (a.elapsed_time(b) / 1e3) / 1e6, 1)
+    return out
+
+
 def relmse(x, r):
     import numpy as np
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
@@ -273,27 +391,16 @@ def bench_serving(ctx, pool, nreq, capacity_samples_s, load, seed, log_fn):
     t_first = min(r.submit_ns for r in results)
     makespan = (t_end - t_first) * 1e-9
     lat = np.array([r.latency_s for r in results])
-    # relMSE of the four smallest requests against 16x their spp (another key)
+    # relMSE of the four smallest requests against converged renders
     small = sorted(range(nreq), key=lambda i: reqs[i][1].width * reqs[i][1].height * reqs[i][1].spp)[:4]
-    ref_jobs = []
-    for i in small:
-        ds, p = reqs[i]
-        from muon_amd import _lib
-        q = _lib.RenderParams.from_buffer_copy(p)
-        q.spp = p.spp * 16
-        ref_jobs.append((ds, q, 1000 + i))
-    refs, _, _ = render_batch(ctx, ref_jobs)
-    rel = []
-    for i, ref in zip(small, refs):
-        p = reqs[i][1]
-        rel.append(round(relmse(outs[i].cpu().numpy() / p.spp, ref.cpu().numpy() / (p.spp * 16)), 5))
+    rel = relmse_vs_converged(ctx, [(reqs[i][0], reqs[i][1], outs[i]) for i in small], 9000)
     return {"requests": nreq, "rate_per_s": round(rate, 3), "load": load,
             "images_per_s": round(nreq / makespan, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "mean_latency_s": round(float(lat.mean()), 3),
             "mean_queue_s": round(float(np.mean([r.queue_s for r in results])), 3),
             "mrays_per_s": round(sum(stats[k] - warm[k] for k in ("primary_rays", "secondary_rays", "shadow_rays"))
                                  / makespan / 1e6, 1),
-            "relmse_vs_16x_spp": rel, "relmse_requests": [f"{reqs[i][1].width}^2x{reqs[i][1].spp}" for i in small]}
+            "relmse_vs_converged": rel}
 
 
 def main():
@@ -482,17 +589,34 @@ def main():
         del ds0
 
     batched = serving = None
+    admission = None
     if not args.no_batch:
-        pool = build_pool(ctx, log)
-        batched, cap = bench_batched(ctx, pool, args.batch_jobs, 7, log)
+        admission = bench_admission(ctx, log)
+        log(f"bench: admission {admission}")
+        # the batched configuration: one independent scene per job (a
+        # displaced icosphere or a sphere field, primitives log-uniform in
+        # [1k, 2M]), BVHs built on the device at admission
+        from muon_amd import workloads
+        specs = workloads.independent_scene_specs(args.batch_jobs, seed=3)
+        t0 = time.perf_counter()
+        pool = workloads.build_independent_pool(ctx, specs, lambda m: log("bench: " + m))
+        pool_s = time.perf_counter() - t0
+        jobs = workloads.independent_jobs(pool, seed=7)
+        batched, cap = bench_batched(ctx, pool, len(jobs), 7, log, jobs=jobs)
+        batched["scene_pool"] = {"scenes": len(pool), "prims": int(sum(i["num_prims"] for *_, i in pool)),
+                                 "icospheres": sum(k == "ico" for k, *_ in pool),
+                                 "sphere_fields": sum(k == "spheres" for k, *_ in pool),
+                                 "admission_s": round(pool_s, 2),
+                                 "device_build_ms": round(sum(i["build_ms"] for *_, i in pool), 1)}
         log(f"bench: batched {batched}")
+        quality_check = converged_cross_check(ctx, orc, cpu_threads, log)
         small, _ = bench_batched(ctx, pool, 128, 8, log, res_choices=(128,), spp_range=(4, 16))
         log(f"bench: batched small jobs {small}")
         batched["small_jobs"] = {k: small[k] for k in ("jobs", "samples", "batched_s", "sequential_s", "images_per_s",
-                                                        "sequential_images_per_s", "speedup", "identical_to_plain")}
-        pool_desc = [f"{k}:{i['num_prims']}" for k, _, _, _, i in pool]
-        batched["scenes"] = pool_desc
-        del pool
+                                                        "sequential_images_per_s", "speedup", "identical_to_plain",
+                                                        "relmse_vs_converged")}
+        batched["converged_cross_check"] = quality_check
+        del pool, jobs
         # serving: its own 8 resident scenes (about 5.7M primitives); the
         # request rate is a load fraction of the batched throughput measured
         # on these scenes with the request mix
@@ -515,7 +639,7 @@ def main():
                    "rays_per_step": args.rays, "prims": info["num_prims"], "bvh_depth": info["bvh_depth"],
                    "hit_fraction": round(hits / args.rays, 4), "wall_s": round(wall, 3)},
         "roofline": roofline, "cpu_baseline": cpu, "autotune": tune, "render": render, "batched": batched,
-        "serving": serving,
+        "serving": serving, "admission": admission,
     }
     if cpu:
         out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)

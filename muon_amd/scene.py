@@ -133,14 +133,18 @@ class DeviceScene:
 
     @classmethod
     def from_arrays(cls, ctx, host_scene, **replace):
-        """A device scene from the host scene's descriptor with some arrays
-        replaced (numpy arrays of the descriptor's element type and length),
-        e.g. materials=... with other BRDF tags. The arrays are read during
-        the call only."""
+        """A device scene from the host scene's descriptor with some fields
+        replaced: counts (ints, e.g. num_prims) and arrays (numpy arrays of the
+        descriptor's element type and length, or None for an optional array),
+        e.g. materials=... with other BRDF tags, or more primitives. The
+        arrays are read during the call only."""
         desc = _lib.SceneDesc.from_buffer_copy(host_scene.desc)
         keep = []
-        for name, arr in replace.items():
-            arr = np.ascontiguousarray(arr)
+        for name, val in replace.items():
+            if val is None or isinstance(val, int):
+                setattr(desc, name, val)
+                continue
+            arr = np.ascontiguousarray(val)
             keep.append(arr)
             setattr(desc, name, arr.ctypes.data)
         return cls(ctx, host_scene, desc)

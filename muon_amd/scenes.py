@@ -12,6 +12,7 @@ path and the CPU restatement parse identical inputs.
 
 Floats are written with 9 significant digits, which round-trips float32.
 """
+import functools
 import math
 
 import numpy as np
@@ -100,6 +101,7 @@ def cornell_text(width=256, height=256, spp=16, max_depth=5, seed=0):
 
 
 # ---------------------------------------------------------------- icosphere
+@functools.lru_cache(maxsize=8)
 def icosphere(level):
     """Unit icosphere: (vertices [V,3] float64, faces [F,3] int64); F = 20*4^level."""
     t = (1.0 + 5 ** 0.5) / 2.0
@@ -174,21 +176,45 @@ def _fmt_rows(arr):
     return [" ".join(F % x for x in row) for row in arr.astype(np.float64).tolist()]
 
 
-def icosphere_text(width=512, height=512, spp=64, max_depth=8, level=7, seed=0, radius=50.0, render_seed=None):
+def icosphere_text(width=512, height=512, spp=64, max_depth=8, level=7, seed=0, radius=50.0, render_seed=None,
+                   mesh_file=None):
     """Displaced icosphere emitted at object radius 50 and placed with
     `translate 0 0.6 0; scale 0.008`: at world scale every triangle's
     2*area would be below the reference's 1e-4 parallel threshold
     (muon/objects.cc:104-108) and the mesh would be invisible. `seed` seeds
-    the displacement and, unless `render_seed` is given, the render."""
+    the displacement and, unless `render_seed` is given, the render. With
+    `mesh_file` the mesh is written there as binary PLY and the scene loads
+    it (`load`) instead of listing its vertices and triangles."""
     verts, faces = displaced_icosphere(level, radius, 0.05, seed)
     lines = [f"# displaced icosphere, level {level}, {len(faces)} tris"]
     cornell_header(lines, width, height, spp, max_depth, seed if render_seed is None else render_seed)
     cornell_walls(lines)
     lines += ["push_transform", "translate 0 0.6 0", "scale 0.008 0.008 0.008", "brdf ggx", "roughness 0.3",
               "diffuse 0.1 0.1 0.1", "specular 0.6 0.6 0.6"]
-    _mesh_lines(lines, verts, faces)
+    if mesh_file is None:
+        _mesh_lines(lines, verts, faces)
+    else:
+        write_ply(mesh_file, verts, faces)
+        lines.append(f"load {mesh_file}")
     lines.append("pop_transform")
     return "\n".join(lines) + "\n"
+
+
+def write_ply(path, verts, faces):
+    """Binary little-endian PLY: float x y z per vertex, a uchar-counted int
+    list per face."""
+    verts = np.ascontiguousarray(verts, "<f4")
+    faces = np.asarray(faces, np.int64)
+    head = (f"ply\nformat binary_little_endian 1.0\nelement vertex {len(verts)}\nproperty float x\n"
+            f"property float y\nproperty float z\nelement face {len(faces)}\n"
+            "property list uchar int vertex_indices\nend_header\n").encode()
+    rec = np.zeros(len(faces), dtype=[("n", "u1"), ("i", "<i4", (3,))])
+    rec["n"] = 3
+    rec["i"] = faces
+    with open(path, "wb") as f:
+        f.write(head)
+        f.write(verts.tobytes())
+        f.write(rec.tobytes())
 
 
 def icosphere_world_bounds(level=7, seed=0, radius=50.0):
@@ -198,21 +224,60 @@ def icosphere_world_bounds(level=7, seed=0, radius=50.0):
 
 
 # ---------------------------------------------------------------- sphere field
-def sphere_field_text(width=1024, height=1024, spp=32, max_depth=8, n=10000, seed=0):
+def sphere_field_extent(n, scaled):
+    """Half-size of the sphere field's box and its number of emitters: the
+    configuration's 10 and 8; a density-scaled field of more than 10,000
+    spheres grows both with the count, so that spheres and emitters keep the
+    10,000-sphere field's density (its emitters are not buried)."""
+    if not scaled or n <= 10000:
+        return 10.0, 8
+    return 10.0 * (n / 10000.0) ** (1.0 / 3.0), int(round(8 * n / 10000.0))
+
+
+def sphere_field_draw(n, seed, half=10.0):
+    """The sphere field's seeded draws: centres, radii, material kinds
+    (0 Lambert, 1 Phong, 2 GGX), colours, Phong exponents, GGX roughness."""
     rng = np.random.default_rng(seed)
-    c = rng.uniform(-10, 10, (n, 3))
+    c = rng.uniform(-half, half, (n, 3))
     r = np.exp(rng.uniform(math.log(0.05), math.log(0.5), n))
     kind = rng.integers(0, 3, n)
     col = rng.uniform(0.2, 0.8, (n, 3))
     expo = np.exp(rng.uniform(math.log(10), math.log(1000), n))
     rough = rng.uniform(0.05, 0.8, n)
-    lines = [
-        f"# sphere field: {n} spheres + 8 emitters",
+    return rng, c, r, kind, col, expo, rough
+
+
+def sphere_field_prologue(width, height, spp, max_depth, n, seed, half=10.0, emitters=8):
+    cam = "camera 0 0 32  0 0 0  0 1 0  45" if half == 10.0 else f"camera 0 0 {_fmt(3.2 * half)}  0 0 0  0 1 0  45"
+    return [
+        f"# sphere field: {n} spheres + {emitters} emitters",
         f"random_seed {seed}", f"film_size {width} {height}", "integrator pathtracer",
         "importance_sampling brdf", "next_event_estimation off", "russian_roulette off",
-        f"max_depth {max_depth}", f"pixel_samples {spp}", "camera 0 0 32  0 0 0  0 1 0  45",
+        f"max_depth {max_depth}", f"pixel_samples {spp}", cam,
         "ambient 0 0 0", "emission 0 0 0",
     ]
+
+
+def sphere_field_emitters(rng, half=10.0, count=8):
+    """The emitting spheres (radius 0.5), drawn after the field's spheres."""
+    lines = ["brdf lambertian", "diffuse 0 0 0", "specular 0 0 0"]
+    ec = rng.uniform(-0.9 * half, 0.9 * half, (count, 3))
+    ecol = rng.uniform(0.5, 1.0, (count, 3))
+    for i in range(count):
+        lines.append("emission " + _fmt(*(20 * ecol[i])))
+        lines.append("sphere " + _fmt(ec[i, 0], ec[i, 1], ec[i, 2], 0.5))
+    return lines
+
+
+def sphere_field_text(width=1024, height=1024, spp=32, max_depth=8, n=10000, seed=0, scaled=False):
+    """The sphere field: n spheres uniform in a box of half-size 10, radii
+    log-uniform in [0.05, 0.5], a third each Lambert / Phong / GGX, lit by 8
+    emitting spheres (NEE off: the reference cannot NEE-sample sphere
+    emitters). `scaled`: box and emitter count grow with n beyond 10,000
+    (sphere_field_extent)."""
+    half, ne = sphere_field_extent(n, scaled)
+    rng, c, r, kind, col, expo, rough = sphere_field_draw(n, seed, half)
+    lines = sphere_field_prologue(width, height, spp, max_depth, n, seed, half, ne)
     for i in range(n):
         if kind[i] == 0:
             lines += ["brdf lambertian", "specular 0 0 0", "diffuse " + _fmt(*col[i])]
@@ -223,12 +288,7 @@ def sphere_field_text(width=1024, height=1024, spp=32, max_depth=8, n=10000, see
             lines += ["brdf ggx", "diffuse " + _fmt(*(0.1 * col[i])), "specular 0.6 0.6 0.6",
                       "roughness " + _fmt(rough[i])]
         lines.append("sphere " + _fmt(c[i, 0], c[i, 1], c[i, 2], r[i]))
-    lines += ["brdf lambertian", "diffuse 0 0 0", "specular 0 0 0"]
-    ec = rng.uniform(-9, 9, (8, 3))
-    ecol = rng.uniform(0.5, 1.0, (8, 3))
-    for i in range(8):
-        lines.append("emission " + _fmt(*(20 * ecol[i])))
-        lines.append("sphere " + _fmt(ec[i, 0], ec[i, 1], ec[i, 2], 0.5))
+    lines += sphere_field_emitters(rng, half, ne)
     return "\n".join(lines) + "\n"
 
 
@@ -297,16 +357,17 @@ def pool_scene_text(kind, size, seed=0):
     return sphere_field_text(256, 256, 16, 8, n=size, seed=seed)
 
 
-def random_pose(params, kind, rng, width, height):
+def random_pose(params, kind, rng, width, height, field_half=10.0):
     """A seeded camera pose: the eye on a sphere around the scene's centre,
     looking at it (inside the enclosure, radius 0.9 around the icosphere;
-    radius 32 around the sphere field), elevation within +-35 degrees."""
+    radius 3.2 x the box half-size, 32 for the configuration's field, around
+    the sphere field), elevation within +-35 degrees."""
     az = rng.uniform(0.0, 2.0 * math.pi)
     el = math.radians(rng.uniform(-35.0, 35.0))
     dirv = (math.sin(az) * math.cos(el), math.sin(el), math.cos(az) * math.cos(el))
     if kind == "ico":
         c, rad, fov = (0.0, 0.6, 0.0), 0.9, 45.0
     else:
-        c, rad, fov = (0.0, 0.0, 0.0), 32.0, 45.0
+        c, rad, fov = (0.0, 0.0, 0.0), 3.2 * field_half, 45.0
     eye = tuple(c[i] + rad * dirv[i] for i in range(3))
     return set_camera(params, eye, c, fov_deg=fov, width=width, height=height)

@@ -6,9 +6,11 @@
   render keys equals the oracle's over 64 render seeds within 5 standard
   errors (reduced film 32x32, 16 spp; tests/golden/make_c1_fixture.py wrote
   the CPU side);
-* batched: the benchmark's 64-job mix (muon_amd.workloads.batched_jobs over
-  the resident pool scenes.POOL: resolution 128^2..1024^2, spp 4..64, depth
-  8) rendered as one batch equals the 64 plain renders bit for bit;
+* batched: the benchmark's 64-job mix (one independent scene per job, a
+  displaced icosphere or a sphere field with primitives log-uniform in
+  [1k, 2M], about 17.6M in all, BVHs built on the device; resolution
+  128^2..1024^2, spp 4..64, depth 8: muon_amd.workloads) rendered as one
+  batch equals the 64 plain renders bit for bit;
 * served: the benchmark's request mix (workloads.served_requests, 48
   requests of 256^2..1024^2 at 8..128 spp) over the 8 resident serving scenes
   (scenes.SERVE_POOL, about 5.7M primitives), submitted as a Poisson stream
@@ -60,9 +62,11 @@ def test_c1_icosphere_render_unbiased(ctx):
 
 
 def test_c3_batched_mix_equals_plain(ctx):
-    pool = workloads.build_pool(ctx, scenes.POOL)
-    jobs = workloads.batched_jobs(pool, 64, seed=7)
-    assert len({id(ds) for ds, _, _ in jobs}) >= 6
+    pool = workloads.build_independent_pool(ctx, workloads.independent_scene_specs(64, seed=3))
+    prims = [i["num_prims"] for *_, i in pool]
+    assert min(prims) >= 1000 and max(prims) <= 2_000_100 and all(i["device_built"] for *_, i in pool)
+    jobs = workloads.independent_jobs(pool, seed=7)
+    assert len({id(ds) for ds, _, _ in jobs}) == 64
     samples = sum(p.width * p.height * p.spp for _, p, _ in jobs)
     film = sum(p.width * p.height * 24 + 256 for _, p, _ in jobs)
     cfg = engine_config(max_jobs=128, film_budget_bytes=film + (1 << 20), iterations_per_poll=4)

@@ -294,3 +294,39 @@ def test_ply_property_less_element_with_huge_count_is_skipped(tmp_path):
     (tmp_path / "s.muon").write_text(HEAD + "load t.ply\n" + TAIL)
     hs = HostScene.from_file(str(tmp_path / "s.muon"))
     assert hs.warnings == 0 and hs.desc.num_prims == 2
+
+
+def test_icosphere_from_binary_ply_equals_inline(tmp_path):
+    """The benchmark's icospheres loaded from binary PLY (scenes.write_ply)
+    parse to the same primitives, transforms and materials as the inline
+    mesh text."""
+    from muon_amd import scenes
+    inline = HostScene.from_text(scenes.icosphere_text(64, 64, 4, 8, level=4, seed=9))
+    path = str(tmp_path / "ico.ply")
+    loaded = HostScene.from_text(scenes.icosphere_text(64, 64, 4, 8, level=4, seed=9, mesh_file=path),
+                                 device_bvh=True)
+    # (one warning each: `random_seed` falls through into `film_size`, muon/parser.cc:267-276)
+    assert loaded.warnings == inline.warnings and loaded.desc.num_nodes == 0
+    a, b = inline.arrays(), loaded.arrays()
+    for k in ("prim_type", "prim_geom", "prim_transform", "prim_material", "prim_light", "transforms", "materials",
+              "lights"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_sphere_field_arrays_equal_the_text_scene():
+    """The benchmark's sphere fields handed over as arrays hold the spheres,
+    transforms and material records the parser makes of the same field as
+    text (the arrays put the 8 emitters first)."""
+    from muon_amd import scenes, workloads
+    for n, ne in ((700, 8), (25000, 20)):  # below and above the density scaling
+        text = HostScene.from_text(scenes.sphere_field_text(64, 64, 4, 8, n=n, seed=4, scaled=True)).arrays()
+        hs, f = workloads.sphere_field_arrays(n, 4, 64, 64, 4, 8)
+        assert f["num_prims"] == n + ne and hs.desc.num_prims == ne
+        geom = f["prim_geom"].reshape(-1, 12)
+        mats = f["materials"].reshape(-1, 16)
+        assert np.array_equal(geom[ne:], text["prim_geom"][:n])
+        assert np.array_equal(geom[:ne], text["prim_geom"][n:])
+        tx = text["transforms"]
+        assert np.array_equal(hs.arrays()["transforms"][f["prim_transform"][ne:]], tx[text["prim_transform"][:n]])
+        assert np.array_equal(mats[f["prim_material"][ne:]], text["materials"][text["prim_material"][:n]])
+    assert (f["prim_light"] == -1).all() and (f["prim_type"] == 1).all()
