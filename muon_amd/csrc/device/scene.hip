@@ -72,8 +72,10 @@ std::vector<Span> SubtreeSpans(const muonx_scene_desc *d) {
 // range), whose references then carry their count (kLeafCountShift); with
 // `merge` 0 leaf references are leaf bit | first primitive. Returns the nodes
 // (8 float4 each, pre-order) and in `max_stack` a bound on the stack entries
-// a traversal can hold: the largest sum of (children - 1) along a
-// root-to-leaf path.
+// a traversal can hold: 3 per wide node along the deepest root-to-leaf path
+// (a step pushes at most 3 children; the quantized nodes' empty slots, which
+// the traversal does not skip by count, can add pushes on nodes of fewer
+// than 4 children).
 std::vector<float4> CollapseWide(const muonx_scene_desc *d, const std::vector<Span> &span, uint32_t merge,
                                  uint32_t *max_stack) {
   auto is_leaf = [&](int32_t c) {
@@ -119,7 +121,7 @@ std::vector<float4> CollapseWide(const muonx_scene_desc *d, const std::vector<Sp
     }
     float lo[3][4], hi[3][4];
     uint32_t ref[4];
-    const uint32_t pushes = it.pushes + uint32_t(n - 1);
+    const uint32_t pushes = it.pushes + 3u;
     for (int k = 0; k < 4; ++k) {
       if (k >= n) {
         for (int a = 0; a < 3; ++a) lo[a][k] = hi[a][k] = 0.0f;
@@ -173,8 +175,13 @@ double PlaneValue(float origin, uint32_t q, int e) { return double(origin) + std
 // child's float box in exact arithmetic (so also after rounding the plane to
 // float, which is monotonic); the device's slab test is conservative against
 // these real-valued planes, so a traversal over the quantized nodes visits a
-// superset of the boxes.
-std::vector<uint4> QuantizeWide(const std::vector<float4> &w) {
+// superset of the boxes. An empty slot (k >= count) gets the inverted box
+// lo = 255, hi = 0 on every axis, which the slab test rejects unless the
+// node is smaller than about 2^-18 of its distance from the ray origin, and
+// the reference `empty_ref`, a leaf of the sentinel primitive that never
+// hits, so that a ray which does pass it only tests that primitive; the
+// traversal therefore needs no per-child count check.
+std::vector<uint4> QuantizeWide(const std::vector<float4> &w, uint32_t empty_ref) {
   const size_t nn = w.size() / 8;
   std::vector<uint4> out(nn * 4);
   for (size_t i = 0; i < nn; ++i) {
@@ -227,14 +234,20 @@ std::vector<uint4> QuantizeWide(const std::vector<float4> &w) {
         if (ok || e >= 127) break;
       }
       ex[a] = e;
+      for (uint32_t k = n; k < 4; ++k) {
+        qlo[a][k] = 255u;
+        qhi[a][k] = 0u;
+      }
     }
+    uint32_t refs[4] = {FloatBits(q[6].x), FloatBits(q[6].y), FloatBits(q[6].z), FloatBits(q[6].w)};
+    for (uint32_t k = n; k < 4; ++k) refs[k] = empty_ref;
     auto pack = [](const uint32_t v[4]) { return v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24); };
     uint4 *o = &out[i * 4];
     o[0] = make_uint4(FloatBits(origin[0]), FloatBits(origin[1]), FloatBits(origin[2]),
                       uint32_t(ex[0] + 127) | (uint32_t(ex[1] + 127) << 8) | (uint32_t(ex[2] + 127) << 16) | (n << 24));
     o[1] = make_uint4(pack(qlo[0]), pack(qlo[1]), pack(qlo[2]), pack(qhi[0]));
     o[2] = make_uint4(pack(qhi[1]), pack(qhi[2]), 0u, 0u);
-    o[3] = make_uint4(FloatBits(q[6].x), FloatBits(q[6].y), FloatBits(q[6].z), FloatBits(q[6].w));
+    o[3] = make_uint4(refs[0], refs[1], refs[2], refs[3]);
   }
   return out;
 }
@@ -486,7 +499,9 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *din, m
   s->ctx = ctx;
   uint64_t total = 0;
   int rc = MUONX_OK;
-  const std::vector<uint4> qnodes = QuantizeWide(wnodes);
+  // empty slots point at the sentinel primitive np (a one-primitive leaf in
+  // either leaf encoding: count field 0, and the sentinel carries the end flag)
+  const std::vector<uint4> qnodes = QuantizeWide(wnodes, kLeafBit | np);
   std::vector<uint32_t> lprims;
   for (uint32_t k = 0; k < np; ++k)
     if (plight[k] >= 0) lprims.push_back(k);

@@ -557,7 +557,13 @@ __device__ __forceinline__ float plane_q(uint32_t word, int k) { return float((w
 // the per-axis slack keeps a ray that runs in an exit plane inside the box
 // (B is then nonzero unless the child is flat at the node's origin, and a
 // flat child holds only primitives parallel to that ray, which never hit).
-template <bool COUNT, class Stack>
+// Empty slots hold an inverted box and a sentinel leaf (the host's
+// quantizer), so no slot is skipped by the node's child count.
+// ANY (an any-hit walk): the children are not ordered by distance. Until the
+// walk returns at its first hit, `best` stays the ray's tmax, so every
+// pushed child passes the pop test anyway; the hit children are moved ahead
+// of the missed ones in child order by flags alone and pushed with key 0.
+template <bool COUNT, bool ANY, class Stack>
 __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
                                            uint32_t &ref, TraceCounters &cnt) {
   const uint4 *np = s.qnodes + 4 * size_t(ref);
@@ -597,42 +603,64 @@ __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay
     const float l = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f)) * kWidenLo;
     const float hh = fminf(fminf(bx, by), bz) * kWidenHi;
     t[k] = l;
-    h[k] = l <= fminf(hh, bm) && uint32_t(k) < n;
+    h[k] = l <= fminf(hh, bm);
   }
   use_now(rf);
-  float k0 = h[0] ? t[0] : kInf, k1 = h[1] ? t[1] : kInf, k2 = h[2] ? t[2] : kInf, k3 = h[3] ? t[3] : kInf;
-  uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
   const int nhit = int(h[0]) + int(h[1]) + int(h[2]) + int(h[3]);
   if (COUNT) {
     cnt.btests += n;
     cnt.bhits += uint32_t(nhit);
   }
   if (nhit == 0) return false;
-  auto cswap = [](float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
-    const bool sw = kb < ka;
-    const float tk = sw ? kb : ka;
-    const uint32_t tr = sw ? rb : ra;
-    kb = sw ? ka : kb;
-    rb = sw ? ra : rb;
-    ka = tk;
-    ra = tr;
-  };
-  cswap(k0, r0, k1, r1);
-  cswap(k2, r2, k3, r3);
-  cswap(k0, r0, k2, r2);
-  cswap(k1, r1, k3, r3);
-  cswap(k1, r1, k2, r2);
-  stack.push_last(nhit - 1, make_uint2(r1, __float_as_uint(k1)), make_uint2(r2, __float_as_uint(k2)),
-                  make_uint2(r3, __float_as_uint(k3)));
-  ref = r0;
-  return true;
+  uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+  if constexpr (ANY) {
+    bool h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
+    // moves a hit child in front of a missed one (a 0/1 sorting network)
+    auto cpart = [](bool &ha, uint32_t &ra, bool &hb, uint32_t &rb) {
+      const bool sw = hb && !ha;
+      const uint32_t tr = sw ? rb : ra;
+      rb = sw ? ra : rb;
+      ra = tr;
+      const bool o = ha || hb;
+      hb = ha && hb;
+      ha = o;
+    };
+    cpart(h0, r0, h1, r1);
+    cpart(h2, r2, h3, r3);
+    cpart(h0, r0, h2, r2);
+    cpart(h1, r1, h3, r3);
+    cpart(h1, r1, h2, r2);
+    stack.push_last(nhit - 1, make_uint2(r1, 0u), make_uint2(r2, 0u), make_uint2(r3, 0u));
+    ref = r0;
+    return true;
+  } else {
+    float k0 = h[0] ? t[0] : kInf, k1 = h[1] ? t[1] : kInf, k2 = h[2] ? t[2] : kInf, k3 = h[3] ? t[3] : kInf;
+    auto cswap = [](float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
+      const bool sw = kb < ka;
+      const float tk = sw ? kb : ka;
+      const uint32_t tr = sw ? rb : ra;
+      kb = sw ? ka : kb;
+      rb = sw ? ra : rb;
+      ka = tk;
+      ra = tr;
+    };
+    cswap(k0, r0, k1, r1);
+    cswap(k2, r2, k3, r3);
+    cswap(k0, r0, k2, r2);
+    cswap(k1, r1, k3, r3);
+    cswap(k1, r1, k2, r2);
+    stack.push_last(nhit - 1, make_uint2(r1, __float_as_uint(k1)), make_uint2(r2, __float_as_uint(k2)),
+                    make_uint2(r3, __float_as_uint(k3)));
+    ref = r0;
+    return true;
+  }
 }
 
 // One internal-node step of the fast mode for node format NODES.
-template <int NODES, bool COUNT, class Stack>
+template <int NODES, bool COUNT, bool ANY, class Stack>
 __device__ __forceinline__ bool fast_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
                                           uint32_t &ref, TraceCounters &cnt) {
-  if constexpr (NODES == kNodesWideQ) return qnode_step<COUNT>(s, r, best, stack, ref, cnt);
+  if constexpr (NODES == kNodesWideQ) return qnode_step<COUNT, ANY>(s, r, best, stack, ref, cnt);
   else if constexpr (NODES == kNodesWide) return wnode_step<COUNT>(s, r, best, stack, ref, cnt);
   else return node_step<COUNT, false>(s, r, best, stack, ref, cnt);
 }
@@ -650,7 +678,7 @@ __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const Tra
     while (true) {
       bool alive = true;
       while (!(ref & kLeafBit)) {
-        if (!fast_step<NODES, COUNT>(s, r, best, stack, ref, cnt) && !pop_next(stack, best, ref)) {
+        if (!fast_step<NODES, COUNT, ANY>(s, r, best, stack, ref, cnt) && !pop_next(stack, best, ref)) {
           alive = false;
           break;
         }
@@ -662,7 +690,7 @@ __device__ __forceinline__ int32_t trace_bvh4_s(const DevSceneView &s, const Tra
   } else {
     while (true) {
       if (!(ref & kLeafBit)) {
-        if (fast_step<NODES, COUNT>(s, r, best, stack, ref, cnt)) continue;
+        if (fast_step<NODES, COUNT, ANY>(s, r, best, stack, ref, cnt)) continue;
       } else {
         if (leaf_step<ANY, COUNT, true>(s, r, best, ref, oc, hit_prim, cnt)) return hit_prim;
       }
@@ -703,7 +731,7 @@ struct LaneTrace {
   template <class Stack>
   __device__ __forceinline__ bool round(const DevSceneView &s, Stack &st, TraceCounters &cnt) {
     while (!(ref & kLeafBit)) {
-      if (!fast_step<WIDTH, COUNT>(s, r, best, st, ref, cnt) && !pop_next(st, best, ref)) return true;
+      if (!fast_step<WIDTH, COUNT, ANY>(s, r, best, st, ref, cnt) && !pop_next(st, best, ref)) return true;
     }
     if (leaf_step<ANY, COUNT, WIDTH != kNodesBinary>(s, r, best, ref, oc, hit, cnt)) return true;
     return !pop_next(st, best, ref);
