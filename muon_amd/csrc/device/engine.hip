@@ -131,16 +131,16 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
                 float ldist;
                 if (kind == MUONX_LIGHT_POINT) {
                   const V3 to_light = V3{L[4], L[5], L[6]} - h.pos;
-                  ldir = normalize(to_light);
-                  ldist = length(to_light);
+                  ldir = est_normalize(to_light);
+                  ldist = est_length(to_light);
                   const float att = L[7] + L[8] * ldist + L[9] * ldist * ldist;
-                  col = col / att;
+                  col = est_div(col, att);
                 } else {
                   ldir = V3{L[4], L[5], L[6]};
                   ldist = __builtin_inff();
                 }
                 V3 irr = col;
-                if (ldist < __builtin_inff()) irr = irr / (ldist * ldist);
+                if (ldist < __builtin_inff()) irr = est_div(irr, ldist * ldist);
                 const float cosi = gmax(dot(h.normal, ldir), 0.0f);
                 const V3 contrib = T * (irr * cosi * brdf_eval(m, ldir, rd, h.normal));
                 const size_t sl = k * P + i;
@@ -157,22 +157,22 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
                 continue;
               }
               const QuadLight ql = load_quad(L);
-              const V3 rn = normalize(cross(ql.e0, ql.e1));
-              const float scale = ql.area / float(job.light_samples);
+              const V3 rn = est_normalize(cross(ql.e0, ql.e1));
+              const float scale = est_div(ql.area, float(job.light_samples));
+              const float cell = est_rcp(float(job.strata));
               for (int si = 0; si < job.strata; ++si) {
                 for (int sj = 0; sj < job.strata; ++sj) {
                   for (int sk = 0; sk < job.per_section; ++sk) {
                     const float u = rng_uniform(key, ctr);
                     const float v = rng_uniform(key, ctr);
-                    V3 lp = (ql.corner + (float(si) + u) / float(job.strata) * ql.e0) +
-                            (float(sj) + v) / float(job.strata) * ql.e1;
+                    V3 lp = (ql.corner + ((float(si) + u) * cell) * ql.e0) + ((float(sj) + v) * cell) * ql.e1;
                     lp = lp - kEps * rn;
                     const V3 to_light = lp - shift;
-                    const V3 ldir = normalize(to_light);
-                    const float ldist = length(to_light);
+                    const V3 ldir = est_normalize(to_light);
+                    const float ldist = est_length(to_light);
                     const float cosi = gmax(dot(h.normal, ldir), 0.0f);
                     const float cosl = gmax(dot(rn, ldir), 0.0f);
-                    const float g = cosi * cosl / (ldist * ldist);
+                    const float g = est_div(cosi * cosl, ldist * ldist);
                     V3 c = g * brdf_eval(m, ldir, rd, h.normal);
                     if (mis) {
                       const float lpdf = nee_pdf(s, shift, ldir);
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
         if (job.rr) {
           const float qc = gmin(gmax(gmax(nT.x, nT.y), nT.z), 1.0f);
           if (qc < rng_uniform(key, ctr)) alive = false;
-          else nT = nT / qc;
+          else nT = est_div(nT, qc);
         }
         // Trace() returns 0 past the depth cutoff (muon/integration.cc:29-33)
         if (alive && depth + 1 <= job.cutoff && depth + 1 < kMaxBounces) {
