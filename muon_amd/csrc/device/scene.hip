@@ -64,18 +64,27 @@ std::vector<Span> SubtreeSpans(const muonx_scene_desc *d) {
   return sp;
 }
 
+// SAH weights of the collapse, per unit of surface area: one wide-node step
+// and one primitive test, in the ratio of their instruction counts.
+constexpr float kCollapseNodeCost = 1.0f, kCollapsePrimCost = 0.75f;
+
 // Collapses the binary tree (pre-order arrays of the descriptor) into 4-ary
-// nodes: each wide node starts from a binary node's two children and keeps
-// replacing the internal child of largest surface area by its two children
-// while it has fewer than 4. Leaves are the binary leaves and, when `merge`
-// is not 0, binary subtrees of at most `merge` primitives (one contiguous
-// range), whose references then carry their count (kLeafCountShift); with
-// `merge` 0 leaf references are leaf bit | first primitive. Returns the nodes
-// (8 float4 each, pre-order) and in `max_stack` a bound on the stack entries
-// a traversal can hold: 3 per wide node along the deepest root-to-leaf path
-// (a step pushes at most 3 children; the quantized nodes' empty slots, which
-// the traversal does not skip by count, can add pushes on nodes of fewer
-// than 4 children).
+// nodes of least SAH cost over the binary tree's cuts (the dynamic program
+// of Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs
+// through compressed wide BVHs", HPG 2017, without their leaf formation):
+// S(n, k) is the least cost of the subtree at n as at most k children of
+// one wide node, either one child (a leaf, or a wide node of cost
+// SA(n) * node cost + S(left, i) + S(right, 4 - i)) or the subtrees of its two
+// children sharing the k places; a leaf costs SA * primitives * prim cost.
+// Leaves are the binary leaves and, when `merge` is not 0, binary subtrees
+// of at most `merge` primitives (one contiguous range), which are never split
+// and whose references then carry their count (kLeafCountShift); with
+// `merge` 0 leaf references are leaf bit | first primitive. Children keep the
+// binary tree's left-to-right order. Returns the nodes (8 float4 each,
+// pre-order) and in `max_stack` a bound on the stack entries a traversal can
+// hold: 3 per wide node along the deepest root-to-leaf path (a step pushes at
+// most 3 children; the quantized nodes' empty slots, which the traversal does
+// not skip by count, can add pushes on nodes of fewer than 4 children).
 std::vector<float4> CollapseWide(const muonx_scene_desc *d, const std::vector<Span> &span, uint32_t merge,
                                  uint32_t *max_stack) {
   auto is_leaf = [&](int32_t c) {
@@ -86,6 +95,58 @@ std::vector<float4> CollapseWide(const muonx_scene_desc *d, const std::vector<Sp
     const Span &q = span[size_t(c)];
     return kLeafBit | ((q.count - 1u) << kLeafCountShift) | q.first;
   };
+  // ---- costs, children after parents (pre-order), so bottom-up in reverse
+  const size_t nn = d->num_nodes;
+  std::vector<float> cost(nn * 4);  // cost[4 n + k - 1] = S(n, k)
+  auto S = [&](int32_t n, int k) -> float & { return cost[size_t(n) * 4 + size_t(k - 1)]; };
+  // best (least cost, first on ties) share i of k places for the left child
+  auto best_split = [&](int32_t l, int32_t r, int k, float *c) {
+    int bi = 1;
+    float bc = S(l, 1) + S(r, k - 1);
+    for (int i = 2; i < k; ++i) {
+      const float ci = S(l, i) + S(r, k - i);
+      if (ci < bc) {
+        bc = ci;
+        bi = i;
+      }
+    }
+    *c = bc;
+    return bi;
+  };
+  for (int64_t n = int64_t(nn) - 1; n >= 0; --n) {
+    const int32_t c = int32_t(n);
+    if (is_leaf(c)) {
+      const uint32_t cnt = d->node_count[c] != 0 ? uint32_t(d->node_count[c]) : span[size_t(c)].count;
+      const float lc = SurfaceArea(d->node_bounds + size_t(c) * 6) * float(cnt) * kCollapsePrimCost;
+      for (int k = 1; k <= 4; ++k) S(c, k) = lc;
+      continue;
+    }
+    const int32_t l = d->node_left[c], r = d->node_right[c];
+    float wide;
+    best_split(l, r, 4, &wide);
+    S(c, 1) = SurfaceArea(d->node_bounds + size_t(c) * 6) * kCollapseNodeCost + wide;
+    for (int k = 2; k <= 4; ++k) {
+      float sc;
+      best_split(l, r, k, &sc);
+      S(c, k) = std::min(S(c, 1), sc);
+    }
+  }
+  // the children of a wide node: the subtree at n in at most k places
+  auto gather = [&](auto &&self, int32_t n, int k, std::vector<int32_t> &kids) -> void {
+    if (k == 1 || is_leaf(n)) {
+      kids.push_back(n);
+      return;
+    }
+    float sc;
+    const int i = best_split(d->node_left[n], d->node_right[n], k, &sc);
+    if (!(sc < S(n, 1))) {
+      kids.push_back(n);
+      return;
+    }
+    self(self, d->node_left[n], i, kids);
+    self(self, d->node_right[n], k - i, kids);
+  };
+
   std::vector<float4> out;
   struct Item {
     int32_t bin;     // binary internal node
@@ -96,29 +157,18 @@ std::vector<float4> CollapseWide(const muonx_scene_desc *d, const std::vector<Sp
   out.resize(8);
   todo.push_back({0, 0, 0});
   *max_stack = 1;
+  std::vector<int32_t> kids;
   while (!todo.empty()) {
     const Item it = todo.back();
     todo.pop_back();
-    int32_t c[4] = {d->node_left[it.bin], d->node_right[it.bin], -1, -1};
-    int n = 2;
-    while (n < 4) {
-      int best = -1;
-      float best_sa = -1.0f;
-      for (int k = 0; k < n; ++k) {
-        if (is_leaf(c[k])) continue;
-        const float sa = SurfaceArea(d->node_bounds + size_t(c[k]) * 6);
-        if (sa > best_sa) {
-          best_sa = sa;
-          best = k;
-        }
-      }
-      if (best < 0) break;
-      const int32_t b = c[best];
-      for (int k = n; k > best + 1; --k) c[k] = c[k - 1];
-      c[best] = d->node_left[b];
-      c[best + 1] = d->node_right[b];
-      ++n;
-    }
+    kids.clear();
+    float unused;
+    const int i = best_split(d->node_left[it.bin], d->node_right[it.bin], 4, &unused);
+    gather(gather, d->node_left[it.bin], i, kids);
+    gather(gather, d->node_right[it.bin], 4 - i, kids);
+    const int n = int(kids.size());
+    int32_t c[4] = {-1, -1, -1, -1};
+    for (int k = 0; k < n; ++k) c[k] = kids[size_t(k)];
     float lo[3][4], hi[3][4];
     uint32_t ref[4];
     const uint32_t pushes = it.pushes + 3u;
