@@ -12,9 +12,8 @@
 // children, and the near child (reference order: children[dir[axis] < 0]
 // first, muon/acceleration.cc:77-82,134-138) is chosen from the axis.
 //
-// Wide BVH: the binary tree collapsed to 4-ary nodes (each binary internal
-// node absorbs the larger-area internal children until it has 4), 128 B per
-// node = 8 x float4:
+// Wide BVH: the binary tree collapsed to 4-ary nodes of least SAH cost
+// (wide_bvh.h), 128 B per node = 8 x float4:
 //   w0 = child min.x[0..3]  w1 = child max.x[0..3]
 //   w2 = child min.y[0..3]  w3 = child max.y[0..3]
 //   w4 = child min.z[0..3]  w5 = child max.z[0..3]
@@ -43,18 +42,11 @@
 
 #include <cstdint>
 
+#include "bvh_layout.h"
+
 namespace muonx {
 
-constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kCountMask = 0x1fffffffu;
-// Leaf references of the wide trees when the scene's leaves carry their
-// primitive count (DevSceneView::leaf_counts): leaf bit | (count - 1) << 24 |
-// first primitive, for scenes of fewer than 2^24 primitives and leaves of at
-// most 128. Otherwise leaf bit | first primitive, and a leaf ends at the
-// primitive flagged last (as in the binary tree).
-constexpr uint32_t kLeafCountShift = 24;
-constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
-constexpr uint32_t kLeafMaxCount = 128;
 constexpr uint32_t kMaxLightPrims = 16;
 constexpr uint32_t kSphereBit = 0x80000000u;
 
