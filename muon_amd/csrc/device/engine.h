@@ -59,7 +59,9 @@ struct JobDev {
   uint32_t sample_base;           // first sample index of the slice
   unsigned long long *film;       // W * H * 3 fixed-point sums
   unsigned long long *done;       // finished samples
+  uint32_t clear_film;            // zero the film at admission (the first slice of a render)
 };
+constexpr uint32_t kClearBlocksPerJob = 64;  // blocks per admitted job in k_admit_clear
 
 // Device-side scalars of the engine (one struct so that the host can update
 // the admission window with a single copy).
@@ -131,7 +133,7 @@ struct JobSpec {
   uint32_t scene_slot;
   muonx_render_params params;
   uint32_t key;
-  unsigned long long *film;  // npix * 3 fixed-point sums, zeroed by Admit when clear_film
+  unsigned long long *film;  // npix * 3 fixed-point sums, zeroed before the next launch when clear_film
   uint32_t sample_begin = 0, sample_count = 0;
   bool clear_film = true;
 };
@@ -150,7 +152,7 @@ class Engine {
   // Shadow rays one vertex of a job generates; admission requires <= kmax.
   static int ShadowRaysPerVertex(const muonx_scene *scene, const muonx_render_params &p);
   // Admits a job (its done counter, and with clear_film its film, are
-  // zeroed on the engine stream).
+  // zeroed on the engine stream before the next launch).
   int Admit(const JobSpec &job, uint64_t *seq);
   // Advances the admission window past finished jobs (all of their samples
   // generated and retired).
@@ -187,8 +189,16 @@ class Engine {
 
  private:
   Engine() = default;
-  int LaunchIteration(int cur);
+  // True when no job of the admission window has NEE on.
+  bool PlainWindow() const;
+  int LaunchIteration(int cur, bool plain);
+  // Copies the records of jobs admitted since the last launch to the device
+  // (one copy per contiguous run of slots) and clears their films and done
+  // counters with one kernel; called before every launch.
+  int FlushAdmitted();
   int SyncWindow();
+  // Destroys the cached iteration graphs (they capture the plan and tables).
+  void DropGraphs();
 
   muonx_ctx *ctx_ = nullptr;
   muonx_tuning tuning_{};
@@ -197,11 +207,13 @@ class Engine {
   uint32_t capacity_ = 0, max_jobs_ = 0;
   int kmax_ = 1;
   bool use_graph_ = false;
-  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  // iteration graphs by kernel set (0: NEE / MIS kernels, 1: plain) and queue parity
+  hipGraphExec_t graph_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   uint32_t max_depth_ = 1, max_wstack_ = 1;
   uint64_t device_bytes_ = 0;
   int cur_ = 0;
   uint64_t seq_lo_ = 0, seq_hi_ = 0, total_items_ = 0;
+  uint64_t flushed_hi_ = 0;  // jobs below it are on the device
   uint64_t window_written_[3] = {~0ull, ~0ull, ~0ull};
   TraverseCfg cfg_{};
   EngArgs a_{};

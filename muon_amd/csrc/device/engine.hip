@@ -324,6 +324,92 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
   ray_count(a.c, 3, culled_n);
 }
 
+// PathTracer::Shade without next-event estimation (muon/integration.cc:205-271
+// with NEE off: emission, then ShadeIndirect with Russian roulette) in one
+// kernel, used for an iteration in which no admitted job has NEE on. The
+// split kernels above do the same arithmetic and draw the same random numbers
+// for such a path (shade_light draws none without NEE, shade_bsdf skips its
+// MIS sample), so a path's result does not depend on which pair ran; this
+// one skips the hit-record hand-off through memory and the shadow and MIS
+// traversals, which would have no rays.
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    k_eng_shade_plain(EngArgs a, int cur) {
+  __shared__ uint32_t wsum[kShadeWaves + 1];
+  const uint32_t n = a.c->qcount[cur].v;
+  const float4 *q = a.q_ray[cur];
+  float4 *qn = a.q_ray[cur ^ 1];
+  const PathState &ps = a.st[cur];
+  const PathState &pn = a.st[cur ^ 1];
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+    const uint32_t i = base + threadIdx.x;
+    bool cont = false;
+    uint32_t js = 0, pixel = 0, sample = 0, ctr = 0;
+    int depth = 0;
+    V3 shift{0, 0, 0}, next_dir{0, 0, 0}, T{0, 0, 0};
+    if (i < n) {
+      a.m_hit[i] = -1;  // retire: no MIS term
+      const int32_t prim = a.hit_prim[i];
+      if (prim >= 0) {
+        js = ps.job[i];
+        const JobDev job = a.jobs[js];
+        const DevSceneView s = a.scenes[job.scene];
+        depth = ps.depth[i];
+        const float4 qo = q[2 * i], qd = q[2 * i + 1];
+        const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
+        const HitRecord h = hit_record(s, uint32_t(prim), ro, rd);
+        const int32_t mi = s.prim_mat[prim];
+        shift = h.pos + kEps * h.normal;
+        const Mat m = load_mat(s.materials + size_t(mi) * kMatStride);
+        T = V3{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
+        if (depth >= job.min_depth && !(dot(h.normal, -rd) < 0.0f)) {
+          // emission (muon/integration.cc:257-264)
+          const V3 rad = V3{ps.rad[3 * i], ps.rad[3 * i + 1], ps.rad[3 * i + 2]} + T * m.emission;
+          ps.rad[3 * i] = rad.x;
+          ps.rad[3 * i + 1] = rad.y;
+          ps.rad[3 * i + 2] = rad.z;
+        }
+        pixel = ps.pixel[i];
+        sample = ps.sample[i];
+        const PathKey key = path_key(job.seed, job.key, pixel, sample);
+        ctr = ps.ctr[i];
+        // ShadeIndirect (muon/integration.cc:294-328)
+        V3 dir, nT;
+        float pdf;
+        if (sample_reflection(job.is_mode, m, rd, h.normal, T, key, ctr, dir, pdf, nT)) {
+          bool alive = true;
+          if (job.rr) {
+            const float qc = gmin(gmax(gmax(nT.x, nT.y), nT.z), 1.0f);
+            if (qc < rng_uniform(key, ctr)) alive = false;
+            else nT = est_div(nT, qc);
+          }
+          // Trace() returns 0 past the depth cutoff (muon/integration.cc:29-33)
+          if (alive && depth + 1 <= job.cutoff && depth + 1 < kMaxBounces) {
+            cont = true;
+            next_dir = dir;
+            T = nT;
+          }
+        }
+      }
+    }
+    // a continuing path moves to the next queue with its state; its radiance
+    // follows in retire
+    const uint32_t ni = block_append<kShadeWaves>(cont, &a.c->qcount[cur ^ 1].v, wsum);
+    if (i < n) a.nidx[i] = cont ? ni : 0xffffffffu;
+    if (cont) {
+      qn[2 * ni] = make_float4(shift.x, shift.y, shift.z, 0.0f);
+      qn[2 * ni + 1] = make_float4(next_dir.x, next_dir.y, next_dir.z, 0.0f);
+      pn.job[ni] = js;
+      pn.pixel[ni] = pixel;
+      pn.sample[ni] = sample;
+      pn.ctr[ni] = ctr;
+      pn.depth[ni] = depth + 1;
+      pn.thr[3 * ni] = T.x;
+      pn.thr[3 * ni + 1] = T.y;
+      pn.thr[3 * ni + 2] = T.z;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- shadow (any hit)
 template <int KIND, bool WW, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs sa, int cur) {
@@ -662,6 +748,18 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
   }
 }
 
+// Zeroes the done counter and, when the job asks for it, the film of each
+// job admitted since the last launch: blockIdx.y picks the job (slots from
+// first_slot on, wrapping at max_jobs), blockIdx.x strides over its film.
+__global__ void k_admit_clear(const JobDev *jobs, uint32_t first_slot, uint32_t max_jobs) {
+  const JobDev &j = jobs[(first_slot + blockIdx.y) % max_jobs];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *j.done = 0ull;
+  if (!j.clear_film) return;
+  const size_t n = size_t(j.npix) * 3;
+  for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += size_t(gridDim.x) * blockDim.x)
+    j.film[k] = 0ull;
+}
+
 __global__ void k_film_resolve(const unsigned long long *__restrict__ film, uint32_t n, float *__restrict__ out) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     out[i] = float(double(film[i]) / kFilmScale);
@@ -744,6 +842,7 @@ struct IterLaunch {
   int cur;
   int fetch;   // per-lane fetch threshold, 0 = grid-stride kernels
   bool count;  // traversal kernels count their box and primitive tests
+  bool plain;  // no admitted job has NEE on: one shade kernel, no shadow / MIS rays
   hipStream_t st;
   template <int KIND, bool WW, int WIDTH>
   void run() const {
@@ -755,7 +854,13 @@ struct IterLaunch {
     const StackArgs sa{cfg.spill, cfg.gcap};
     const dim3 grid(cfg.grid), block(kEngBlock);
     const dim3 sgrid(std::max(1, cfg.grid * kEngBlock / kShadeBlock)), sblock(kShadeBlock);
-    if (fetch > 0) {
+    if (plain) {
+      if (fetch > 0)
+        hipLaunchKernelGGL((k_eng_extend_dyn<KIND, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur, fetch);
+      else
+        hipLaunchKernelGGL((k_eng_extend<KIND, WW, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur);
+      hipLaunchKernelGGL(k_eng_shade_plain, sgrid, sblock, 0, st, a, cur);
+    } else if (fetch > 0) {
       hipLaunchKernelGGL((k_eng_extend_dyn<KIND, WIDTH, COUNT>), grid, block, 0, st, a, sa, cur, fetch);
       hipLaunchKernelGGL(k_eng_shade_light, sgrid, sblock, 0, st, a, cur);
       hipLaunchKernelGGL(k_eng_shade_bsdf, sgrid, sblock, 0, st, a, cur);
@@ -854,8 +959,7 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  for (auto &g : graph_)
-    if (g) (void)hipGraphExecDestroy(g);
+  DropGraphs();
   for (auto &ev : window_ev_)
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : done_ev_)
@@ -911,17 +1015,17 @@ int Engine::AddScene(muonx_scene *scene, uint32_t *slot) {
                                     kEngBlock, spill_, cfg, false);
     if (rc != MUONX_OK) return rc;
     cfg_ = cfg;
-    for (auto &g : graph_) {
-      if (g) (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
-  } else {
-    for (auto &g : graph_) {  // the scene table pointer may have moved
-      if (g) (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
   }
+  DropGraphs();  // a new plan, or the scene table pointer may have moved
   return MUONX_OK;
+}
+
+void Engine::DropGraphs() {
+  for (auto &set : graph_)
+    for (auto &g : set) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
 namespace {
@@ -988,19 +1092,39 @@ int Engine::Admit(const JobSpec &js, uint64_t *seq) {
   j.sample_base = js.sample_begin;
   j.film = js.film;
   j.done = done_.as<unsigned long long>() + slot;
+  j.clear_film = js.clear_film ? 1u : 0u;
   h_first_[slot] = j.first_item;
-  if (js.clear_film) MX_HIP(hipMemsetAsync(js.film, 0, size_t(j.npix) * 3 * sizeof(unsigned long long), stream_));
-  MX_HIP(hipMemsetAsync(j.done, 0, sizeof(unsigned long long), stream_));
-  MX_HIP(hipMemcpyAsync(jobs_.as<JobDev>() + slot, &j, sizeof(JobDev), hipMemcpyHostToDevice, stream_));
-  MX_HIP(hipMemcpyAsync(job_first_.as<unsigned long long>() + slot, &h_first_[slot], sizeof(unsigned long long), hipMemcpyHostToDevice,
-                        stream_));
+  // the record reaches the device, and the film and done counter are
+  // cleared, with the other jobs admitted before the next launch (FlushAdmitted)
   *seq = seq_hi_;
   ++seq_hi_;
   total_items_ += j.nitems;
   return MUONX_OK;
 }
 
+int Engine::FlushAdmitted() {
+  if (flushed_hi_ == seq_hi_) return MUONX_OK;
+  const uint64_t lo = flushed_hi_, hi = seq_hi_;
+  // the new slots in the ring: at most two contiguous runs
+  for (uint64_t q = lo; q < hi;) {
+    const uint32_t s0 = uint32_t(q % max_jobs_);
+    const uint32_t run = uint32_t(std::min<uint64_t>(hi - q, max_jobs_ - s0));
+    MX_HIP(hipMemcpyAsync(jobs_.as<JobDev>() + s0, h_jobs_ + s0, sizeof(JobDev) * run, hipMemcpyHostToDevice,
+                          stream_));
+    MX_HIP(hipMemcpyAsync(job_first_.as<unsigned long long>() + s0, h_first_ + s0,
+                          sizeof(unsigned long long) * run, hipMemcpyHostToDevice, stream_));
+    q += run;
+  }
+  hipLaunchKernelGGL(k_admit_clear, dim3(kClearBlocksPerJob, uint32_t(hi - lo)), dim3(256), 0, stream_,
+                     jobs_.as<JobDev>(), uint32_t(lo % max_jobs_), max_jobs_);
+  MX_HIP(hipGetLastError());
+  flushed_hi_ = hi;
+  return MUONX_OK;
+}
+
 int Engine::SyncWindow() {
+  const int frc = FlushAdmitted();
+  if (frc != MUONX_OK) return frc;
   if (window_written_[0] == seq_lo_ && window_written_[1] == seq_hi_ && window_written_[2] == total_items_)
     return MUONX_OK;
   const int k = window_slot_;
@@ -1019,8 +1143,14 @@ int Engine::SyncWindow() {
   return MUONX_OK;
 }
 
-int Engine::LaunchIteration(int cur) {
-  DispatchTraverse(cfg_, IterLaunch{a_, cfg_, cur, tuning_.ray_fetch, tuning_.count_work != 0, stream_});
+bool Engine::PlainWindow() const {
+  for (uint64_t q = seq_lo_; q < seq_hi_; ++q)
+    if (h_jobs_[q % max_jobs_].nee != MUONX_NEE_OFF) return false;
+  return true;
+}
+
+int Engine::LaunchIteration(int cur, bool plain) {
+  DispatchTraverse(cfg_, IterLaunch{a_, cfg_, cur, tuning_.ray_fetch, tuning_.count_work != 0, plain, stream_});
   MX_HIP(hipGetLastError());
   return MUONX_OK;
 }
@@ -1042,11 +1172,15 @@ int Engine::Iterate() {
   }
   int rc = SyncWindow();
   if (rc != MUONX_OK) return rc;
+  // Jobs below seq_lo have no path in flight, so with no NEE job in the
+  // window no path needs the NEE / MIS kernels.
+  const bool plain = PlainWindow();
+  hipGraphExec_t &graph = graph_[plain ? 1 : 0][cur_];
   if (use_graph_) {
-    if (!graph_[cur_]) {
+    if (!graph) {
       hipGraph_t g;
       MX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      rc = LaunchIteration(cur_);
+      rc = LaunchIteration(cur_, plain);
       hipGraph_t captured = nullptr;
       const hipError_t ec = hipStreamEndCapture(stream_, &captured);
       if (rc != MUONX_OK) return rc;
@@ -1055,12 +1189,12 @@ int Engine::Iterate() {
         return MUONX_ERR_HIP;
       }
       g = captured;
-      MX_HIP(hipGraphInstantiate(&graph_[cur_], g, nullptr, nullptr, 0));
+      MX_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
       MX_HIP(hipGraphDestroy(g));
     }
-    MX_HIP(hipGraphLaunch(graph_[cur_], stream_));
+    MX_HIP(hipGraphLaunch(graph, stream_));
   } else {
-    rc = LaunchIteration(cur_);
+    rc = LaunchIteration(cur_, plain);
     if (rc != MUONX_OK) return rc;
   }
   cur_ ^= 1;

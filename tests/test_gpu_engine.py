@@ -68,6 +68,11 @@ def job_list(resident):
 
 
 def test_batch_equals_alone(ctx, resident):
+    """Every job's image in the batch is its image alone, bit for bit. The
+    jobs without NEE (the sphere field, the Cornell box with nee 0) render
+    alone with the single shade kernel and in this batch, which holds NEE
+    jobs, with the split NEE / BSDF kernels, so this also checks that the two
+    kernel sets give the same bits."""
     jobs = job_list(resident)
     ref = [alone(ds, p) for ds, p, _ in jobs]
     accs, results, st = render_batch(ctx, jobs)
