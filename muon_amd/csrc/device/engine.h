@@ -63,6 +63,17 @@ struct JobDev {
 };
 constexpr uint32_t kClearBlocksPerJob = 64;  // blocks per admitted job in k_admit_clear
 
+// One finished render's film and its float output, for k_film_resolve.
+struct FilmResolve {
+  const unsigned long long *film;
+  float *out;
+  uint32_t npix;
+};
+constexpr int kResolveBatch = 16;
+struct FilmResolveBatch {
+  FilmResolve r[kResolveBatch];
+};
+
 // Device-side scalars of the engine (one struct so that the host can update
 // the admission window with a single copy).
 // A counter on a 256-byte line of its own: atomics on one L2 line are
@@ -171,8 +182,9 @@ class Engine {
   const unsigned long long *done_host(int b) const { return h_done_[b]; }
   // Ray counters (primary, secondary, shadow) taken with snapshot `b`.
   const unsigned long long *rays_host(int b) const { return h_done_[b] + max_jobs_; }
-  // Converts a job's fixed-point film to float sums on the engine stream.
-  int ResolveFilm(const unsigned long long *film, uint32_t npix, float *out);
+  // Converts finished renders' fixed-point films to float sums on the engine
+  // stream, up to kResolveBatch renders per launch.
+  int ResolveFilms(const FilmResolve *films, size_t n);
   // Ray counters (primary, secondary, shadow) since the last reset; waits.
   int RayCounts(unsigned long long out[kRayCounters]);
   int ResetRayCounts();

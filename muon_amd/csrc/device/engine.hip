@@ -760,9 +760,13 @@ __global__ void k_admit_clear(const JobDev *jobs, uint32_t first_slot, uint32_t 
     j.film[k] = 0ull;
 }
 
-__global__ void k_film_resolve(const unsigned long long *__restrict__ film, uint32_t n, float *__restrict__ out) {
+// Fixed-point films to float sums for up to kResolveBatch finished renders:
+// blockIdx.y picks the render, blockIdx.x strides over its film.
+__global__ void k_film_resolve(FilmResolveBatch b) {
+  const FilmResolve &r = b.r[blockIdx.y];
+  const uint32_t n = r.npix * 3u;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    out[i] = float(double(film[i]) / kFilmScale);
+    r.out[i] = float(double(r.film[i]) / kFilmScale);
 }
 
 // ---------------------------------------------------------------- host side
@@ -1215,11 +1219,19 @@ int Engine::WaitDone(int b) {
   return MUONX_OK;
 }
 
-int Engine::ResolveFilm(const unsigned long long *film, uint32_t npix, float *out) {
-  const uint32_t n = npix * 3;
-  hipLaunchKernelGGL(k_film_resolve, dim3(std::min<uint32_t>(4096, (n + 255) / 256)), dim3(256), 0, stream_, film, n,
-                     out);
-  MX_HIP(hipGetLastError());
+int Engine::ResolveFilms(const FilmResolve *films, size_t n) {
+  for (size_t k = 0; k < n; k += kResolveBatch) {
+    FilmResolveBatch b{};
+    const uint32_t m = uint32_t(std::min<size_t>(kResolveBatch, n - k));
+    uint32_t most = 0;
+    for (uint32_t q = 0; q < m; ++q) {
+      b.r[q] = films[k + q];
+      most = std::max(most, films[k + q].npix * 3u);
+    }
+    hipLaunchKernelGGL(k_film_resolve, dim3(std::min<uint32_t>(1024, (most + 255) / 256), m), dim3(256), 0,
+                       stream_, b);
+    MX_HIP(hipGetLastError());
+  }
   return MUONX_OK;
 }
 

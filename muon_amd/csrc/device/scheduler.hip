@@ -214,6 +214,7 @@ int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
   (void)finished;
   const unsigned long long *done = eng_->done_host(b);
   if (slices_.empty()) progress_iter_ = iterations_;
+  std::vector<uint64_t> complete;  // renders whose every sample is in the film
   for (auto it = slices_.begin(); it != slices_.end();) {
     Slice &sl = it->second;
     if (sl.admit_snap > snap_id_[b]) {  // snapshot older than the slice's admission
@@ -230,13 +231,26 @@ int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
       ++it;
       continue;
     }
-    SchedJob &j = jobs_.at(sl.ticket);
-    it = slices_.erase(it);
+    const uint64_t ticket = sl.ticket;
+    SchedJob &j = jobs_.at(ticket);
+    it = slices_.erase(it);  // sl is gone from here on
     --j.open_slices;
     if (j.open_slices != 0 || j.next_sample < uint32_t(j.spec.params.spp)) continue;
-    // every sample of the job is in its film
-    int rc = eng_->ResolveFilm(films_.at(j.film_off), j.npix, j.spec.accum_rgb);
+    complete.push_back(ticket);
+  }
+  // their films, resolved together (a few launches for all of them)
+  std::vector<FilmResolve> resolves;
+  resolves.reserve(complete.size());
+  for (uint64_t t : complete) {
+    const SchedJob &j = jobs_.at(t);
+    resolves.push_back(FilmResolve{films_.at(j.film_off), j.spec.accum_rgb, j.npix});
+  }
+  if (!resolves.empty()) {
+    const int rc = eng_->ResolveFilms(resolves.data(), resolves.size());
     if (rc != MUONX_OK) return rc;
+  }
+  for (uint64_t t : complete) {
+    SchedJob &j = jobs_.at(t);
     if (free_events_.empty()) {
       hipEvent_t ev;
       MX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
