@@ -85,6 +85,10 @@ struct DevSceneView {
   const uint32_t *light_prims;
   uint32_t num_light_prims;
   uint32_t cull_mis;
+  // World box around those primitives (min xyz, max xyz), padded so that a
+  // ray that hits one of them passes its slab test: a MIS ray that misses the
+  // box skips their primitive tests.
+  float light_box[6];
 };
 
 }  // namespace muonx

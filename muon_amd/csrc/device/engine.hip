@@ -263,11 +263,14 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
             const TraceRay mr = make_ray(shift, dir);
             ObjCache oc;
             oc.tid = -1;
-            float dmin = __builtin_inff();
-            for (uint32_t k = 0; k < s.num_light_prims; ++k) {
-              float dist;
-              uint32_t flags;
-              if (prim_hit(s, s.light_prims[k], mr, oc, dist, flags) && dist > 0.0f && dist < dmin) dmin = dist;
+            float dmin = __builtin_inff(), blo;
+            const float *lb = s.light_box;
+            if (box_hit(mr, lb[0], lb[3], lb[1], lb[4], lb[2], lb[5], __builtin_inff(), blo)) {
+              for (uint32_t k = 0; k < s.num_light_prims; ++k) {
+                float dist;
+                uint32_t flags;
+                if (prim_hit(s, s.light_prims[k], mr, oc, dist, flags) && dist > 0.0f && dist < dmin) dmin = dist;
+              }
             }
             if (dmin == __builtin_inff()) {
               want_mis = false;

@@ -17,6 +17,53 @@
 namespace muonx {
 namespace {
 
+// World box around the first `n` primitives of `ids` (leaf order), padded by
+// 1e-3 of its diagonal plus 1e-5 of its largest coordinate: the primitive
+// tests run in object space, so a ray that hits one of them passes within
+// rounding of its transformed vertices (a sphere: of its transformed object
+// box). All zeros when n is 0.
+void EmitterBox(const std::vector<float4> &prims, const std::vector<float> &xf, const std::vector<uint32_t> &ids,
+                uint32_t n, float box[6]) {
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (uint32_t q = 0; q < n && q < ids.size(); ++q) {
+    const float4 *p = &prims[size_t(ids[q]) * 3];
+    uint32_t tw;
+    std::memcpy(&tw, &p[0].w, 4);
+    const float *m = &xf[size_t(tw & 0x7fffffffu) * 48];  // column-major M
+    auto add = [&](double x, double y, double z) {
+      const double w[3] = {m[0] * x + m[4] * y + m[8] * z + m[12], m[1] * x + m[5] * y + m[9] * z + m[13],
+                           m[2] * x + m[6] * y + m[10] * z + m[14]};
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], w[a]);
+        hi[a] = std::max(hi[a], w[a]);
+      }
+    };
+    if (tw & kSphereBit) {
+      const double r = p[1].x;
+      for (int c = 0; c < 8; ++c)
+        add(p[0].x + ((c & 1) ? r : -r), p[0].y + ((c & 2) ? r : -r), p[0].z + ((c & 4) ? r : -r));
+    } else {
+      add(p[0].x, p[0].y, p[0].z);
+      add(p[1].x, p[1].y, p[1].z);
+      add(p[2].x, p[2].y, p[2].z);
+    }
+  }
+  if (!(lo[0] <= hi[0])) {
+    for (int a = 0; a < 6; ++a) box[a] = 0.0f;
+    return;
+  }
+  double diag = 0.0, big = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    diag += (hi[a] - lo[a]) * (hi[a] - lo[a]);
+    big = std::max(big, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+  }
+  const double pad = 1e-3 * std::sqrt(diag) + 1e-5 * big + 1e-30;
+  for (int a = 0; a < 3; ++a) {
+    box[a] = std::nextafter(float(lo[a] - pad), -HUGE_VALF);
+    box[a + 3] = std::nextafter(float(hi[a] + pad), HUGE_VALF);
+  }
+}
+
 // Reflectiveness of Phong (muon/materials.cc:113-123) and GGX (:270-293).
 float Reflectiveness(const float *m) {
   const float s = ((0.0f + m[6]) + m[7] + m[8]) / 3.0f;
@@ -315,6 +362,7 @@ extern "C" int muonx_scene_create(muonx_ctx *ctx, const muonx_scene_desc *din, m
     uint32_t ne = 0;
     for (uint32_t k = 0; k < np; ++k) ne += plight[k] >= 0 ? 1u : 0u;
     v.num_light_prims = cull_mis ? ne : 0u;
+    EmitterBox(prims, xf, cull_mis ? lprims : std::vector<uint32_t>{}, v.num_light_prims, v.light_box);
   }
   v.prims = s->prims.as<float4>();
   v.xforms = s->xforms.as<float>();
