@@ -87,7 +87,14 @@ int RunInline(Scheduler *s, const muonx_job *jobs, uint32_t n, muonx_job_result 
   if (rc != MUONX_OK) return rc;
   const uint64_t t0 = NowNs();
   for (uint32_t i = 0; i < n; ++i)
-    if ((rc = s->Submit(jobs[i], i, t0)) != MUONX_OK) return rc;
+    if ((rc = s->Validate(jobs[i])) != MUONX_OK) return rc;
+  // Jobs without NEE first (in their order), then the others: while only the
+  // former are admitted the engine runs its single-shade-kernel iterations.
+  // A job's image does not depend on the order.
+  for (int with_nee = 0; with_nee < 2; ++with_nee)
+    for (uint32_t i = 0; i < n; ++i)
+      if ((jobs[i].params.nee != MUONX_NEE_OFF) == (with_nee != 0) && (rc = s->Submit(jobs[i], i, t0)) != MUONX_OK)
+        return rc;
   std::vector<SchedJob> finished;
   finished.reserve(n);
   while (!s->Idle())
