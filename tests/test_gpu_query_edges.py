@@ -55,8 +55,10 @@ def _run(ds, o, d, tmax=None, exact=False):
 
 
 # The Cornell box's two boxes rest on its floor: their bottom faces and the
-# floor tie for rays that start inside a box and leave through its bottom.
-CORNELL_TIE_FRAC = 5e-3
+# floor tie for rays that start inside a box and leave through its bottom
+# (about 1.1 % of rays spread uniformly through the room, counted with the
+# oracle), and the visit order decides which one is reported.
+CORNELL_TIE_FRAC = 2e-2
 
 
 def _check_closest(orc, o, d, prim, t, pr, tr, exact, max_graze_frac=1e-4):
