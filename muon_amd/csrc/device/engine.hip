@@ -1112,15 +1112,14 @@ int Engine::Admit(const JobSpec &js, uint64_t *seq) {
 int Engine::FlushAdmitted() {
   if (flushed_hi_ == seq_hi_) return MUONX_OK;
   const uint64_t lo = flushed_hi_, hi = seq_hi_;
-  // the new slots in the ring: at most two contiguous runs
-  for (uint64_t q = lo; q < hi;) {
-    const uint32_t s0 = uint32_t(q % max_jobs_);
-    const uint32_t run = uint32_t(std::min<uint64_t>(hi - q, max_jobs_ - s0));
+  SlotRun runs[2];
+  const int nr = SlotRuns(lo, hi, max_jobs_, runs);  // the new slots: at most two runs of the ring
+  for (int r = 0; r < nr; ++r) {
+    const uint32_t s0 = runs[r].first, run = runs[r].count;
     MX_HIP(hipMemcpyAsync(jobs_.as<JobDev>() + s0, h_jobs_ + s0, sizeof(JobDev) * run, hipMemcpyHostToDevice,
                           stream_));
     MX_HIP(hipMemcpyAsync(job_first_.as<unsigned long long>() + s0, h_first_ + s0,
                           sizeof(unsigned long long) * run, hipMemcpyHostToDevice, stream_));
-    q += run;
   }
   hipLaunchKernelGGL(k_admit_clear, dim3(kClearBlocksPerJob, uint32_t(hi - lo)), dim3(256), 0, stream_,
                      jobs_.as<JobDev>(), uint32_t(lo % max_jobs_), max_jobs_);

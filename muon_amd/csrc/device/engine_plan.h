@@ -101,6 +101,25 @@ constexpr uint64_t PathSlotBytes(int kmax) {
   return path + shadow * uint64_t(kmax);
 }
 
+// The job slots of jobs [lo, hi) in the ring of max_jobs slots (slot = seq
+// % max_jobs, hi - lo <= max_jobs) as at most two contiguous runs, in job
+// order: what the admission flush copies to the device. Returns the number
+// of runs.
+struct SlotRun {
+  uint32_t first, count;
+};
+constexpr int SlotRuns(unsigned long long lo, unsigned long long hi, uint32_t max_jobs, SlotRun *out) {
+  int n = 0;
+  for (unsigned long long q = lo; q < hi && n < 2;) {
+    const uint32_t s0 = uint32_t(q % max_jobs);
+    const unsigned long long left = hi - q, room = max_jobs - s0;
+    const uint32_t run = uint32_t(left < room ? left : room);
+    out[n++] = SlotRun{s0, run};
+    q += run;
+  }
+  return n;
+}
+
 }  // namespace muonx
 
 #endif

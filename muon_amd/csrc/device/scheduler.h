@@ -104,7 +104,7 @@ class Scheduler {
   Scheduler() = default;
   int AdmitWork();
   int AdmitSlice(SchedJob &j);
-  int Harvest(int b, std::vector<SchedJob> *finished);
+  int Harvest(int b);
   int FinishResolves(std::vector<SchedJob> *finished, bool wait_all);
   void AdvanceWindow();
 

@@ -210,8 +210,7 @@ int Scheduler::AdmitWork() {
 
 void Scheduler::AdvanceWindow() { eng_->Retire(slices_.empty() ? eng_->seq_hi() : slices_.begin()->first); }
 
-int Scheduler::Harvest(int b, std::vector<SchedJob> *finished) {
-  (void)finished;
+int Scheduler::Harvest(int b) {
   const unsigned long long *done = eng_->done_host(b);
   if (slices_.empty()) progress_iter_ = iterations_;
   std::vector<uint64_t> complete;  // renders whose every sample is in the film
@@ -322,7 +321,7 @@ int Scheduler::Step(std::vector<SchedJob> *finished) {
     if (snap_pending_[pb]) {
       if ((rc = eng_->WaitDone(pb)) != MUONX_OK) return rc;
       snap_pending_[pb] = false;
-      if ((rc = Harvest(pb, finished)) != MUONX_OK) return rc;
+      if ((rc = Harvest(pb)) != MUONX_OK) return rc;
     }
   } else {
     for (int k = 0; k < 2; ++k) {
@@ -330,7 +329,7 @@ int Scheduler::Step(std::vector<SchedJob> *finished) {
       if (!snap_pending_[b]) continue;
       if ((rc = eng_->WaitDone(b)) != MUONX_OK) return rc;
       snap_pending_[b] = false;
-      if ((rc = Harvest(b, finished)) != MUONX_OK) return rc;
+      if ((rc = Harvest(b)) != MUONX_OK) return rc;
     }
   }
   return FinishResolves(finished, slices_.empty() && pending_.empty());

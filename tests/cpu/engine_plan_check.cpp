@@ -214,12 +214,33 @@ static void CheckSlicePolicy() {
   }
 }
 
+// The admission flush's slot runs: every job of [lo, hi) exactly once, in
+// job order, inside the ring, in at most two runs.
+static void CheckSlotRuns(std::mt19937_64 &rng) {
+  for (int rep = 0; rep < 20000; ++rep) {
+    const uint32_t M = 1 + uint32_t(rng() % 300);
+    const unsigned long long lo = rng() % 100000ull + (rep & 1 ? (1ull << 40) : 0ull);
+    const unsigned long long hi = lo + rng() % (M + 1ull);
+    SlotRun runs[2] = {};
+    const int n = SlotRuns(lo, hi, M, runs);
+    CHECK(n >= 0 && n <= 2);
+    CHECK((hi == lo) == (n == 0));
+    unsigned long long q = lo;
+    for (int r = 0; r < n; ++r) {
+      CHECK(runs[r].count > 0 && runs[r].first < M && runs[r].first + runs[r].count <= M);
+      for (uint32_t k = 0; k < runs[r].count; ++k, ++q) CHECK(runs[r].first + k == uint32_t(q % M));
+    }
+    CHECK(q == hi);
+  }
+}
+
 int main() {
   std::mt19937_64 rng(12345);
   CheckLayout();
   CheckWindow(rng);
   CheckRanges(rng);
   CheckSlicePolicy();
+  CheckSlotRuns(rng);
   std::printf("ok\n");
   return 0;
 }

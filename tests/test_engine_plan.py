@@ -3,8 +3,8 @@ the CPU: tests/cpu/engine_plan_check.cpp, built with the host C++ compiler
 against muon_amd/csrc/device/engine_plan.h and sched_policy.h (the headers
 the kernels and the scheduler use), runs randomized checks of the buffer
 layout, the admission window / plan / fill mapping of samples to jobs,
-pixels and queue positions, the film range allocator and the slice choice's
-starvation bound."""
+pixels and queue positions, the film range allocator, the slice choice's
+starvation bound and the job-slot runs the admission flush copies."""
 import os
 import shutil
 import subprocess
