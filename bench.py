@@ -210,7 +210,7 @@ def read_rates_gbs(dev, num_cus):
 
 
 def roofline_of(rays, mean_ms, dev_nodes, dev_prims, ref_nodes, ref_objs, scene_bytes, dev):
-    """Roofline of the closest-hit kernel (see DESIGN.md for the byte models).
+    """Roofline of the closest-hit kernel.
 
     achieved: the device tree's algorithmic bytes per ray, 24 (origin,
     direction) + 8 (prim id, distance) + 64 per quantized 4-wide node fetched
@@ -255,7 +255,7 @@ CONVERGED_SPP = 16384
 
 
 def relmse_vs_converged(ctx, items, key_base):
-    """relMSE (SURVEY's definition: mean over pixels and channels of
+    """relMSE (mean over pixels and channels of
     (x - r)^2 / (r^2 + 1e-2) on the linear film / spp) of each rendered image
     against a converged render of the same job at 16,384 spp with another
     random stream. items: (DeviceScene, RenderParams, accum tensor). Returns
