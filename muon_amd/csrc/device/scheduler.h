@@ -76,7 +76,7 @@ class Scheduler {
   // pending (admitted, finished or unknown). `submit_ns` gets its arrival.
   bool Cancel(uint64_t ticket, uint64_t *submit_ns);
   // One round: admits work, launches `iterations_per_poll` engine
-  // iterations, harvests the previous round's done counters and finished
+  // iterations, harvests the previous poll's done counters and finished
   // resolves. Jobs that completed are appended to `finished`.
   //
   // Admission: a job enters in arrival order once its film fits the film

@@ -316,7 +316,7 @@ int Scheduler::Step(std::vector<SchedJob> *finished) {
     snap_id_[b] = snap_count_++;
     snap_pending_[b] = true;
     snap_ ^= 1;
-    // the previous round's counters, read while this round runs
+    // the previous poll's counters, read while this poll's iterations run
     const int pb = b ^ 1;
     if (snap_pending_[pb]) {
       if ((rc = eng_->WaitDone(pb)) != MUONX_OK) return rc;
