@@ -75,7 +75,7 @@ class Scheduler {
   // Removes a job that is still waiting for admission; false if it is not
   // pending (admitted, finished or unknown). `submit_ns` gets its arrival.
   bool Cancel(uint64_t ticket, uint64_t *submit_ns);
-  // One round: admits work, launches `iterations_per_poll` engine
+  // One poll: admits work, launches `iterations_per_poll` engine
   // iterations, harvests the previous poll's done counters and finished
   // resolves. Jobs that completed are appended to `finished`.
   //
