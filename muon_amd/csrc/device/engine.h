@@ -109,9 +109,10 @@ struct EngCounters {
 
 // Per-path state, struct of arrays over queue positions.
 struct PathState {
-  uint32_t *job, *pixel, *sample, *ctr;
+  uint32_t *job, *pixel, *ctr;
   int32_t *depth;
-  float *rad, *thr;  // 3 per path
+  float *rad, *thr;       // 3 per path
+  uint32_t *key_a, *key_b;  // the sample's random-stream key (path_key), set at fill
 };
 
 struct EngArgs {

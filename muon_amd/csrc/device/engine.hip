@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
           ps.rad[3 * i + 2] = rad.z;
           if (job.nee != MUONX_NEE_OFF) {
             // ShadeDirectNEE (muon/integration.cc:340-466): one shadow ray per light sample
-            const PathKey key = path_key(job.seed, job.key, ps.pixel[i], ps.sample[i]);
+            const PathKey key{ps.key_a[i], ps.key_b[i]};
             uint32_t ctr = ps.ctr[i];
             const bool mis = job.nee == MUONX_NEE_MIS;
             int k = 0;
@@ -222,7 +222,8 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
   for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
     bool cont = false, want_mis = false, culled = false;
-    uint32_t js = 0, pixel = 0, sample = 0, ctr = 0;
+    uint32_t js = 0, pixel = 0, ctr = 0;
+    PathKey key{0u, 0u};
     int depth = 0;
     float mis_tmax = __builtin_inff();
     V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0}, T{0, 0, 0};
@@ -238,8 +239,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
       const float4 qd = q[2 * i + 1];
       const V3 rd{qd.x, qd.y, qd.z};
       pixel = ps.pixel[i];
-      sample = ps.sample[i];
-      const PathKey key = path_key(job.seed, job.key, pixel, sample);
+      key = PathKey{ps.key_a[i], ps.key_b[i]};
       ctr = ps.ctr[i];
       T = V3{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
       if (depth >= job.min_depth && job.nee == MUONX_NEE_MIS) {
@@ -308,8 +308,9 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
       qn[2 * ni + 1] = make_float4(next_dir.x, next_dir.y, next_dir.z, 0.0f);
       pn.job[ni] = js;
       pn.pixel[ni] = pixel;
-      pn.sample[ni] = sample;
       pn.ctr[ni] = ctr;
+      pn.key_a[ni] = key.a;
+      pn.key_b[ni] = key.b;
       pn.depth[ni] = depth + 1;
       pn.thr[3 * ni] = T.x;
       pn.thr[3 * ni + 1] = T.y;
@@ -346,7 +347,8 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
   for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
     bool cont = false;
-    uint32_t js = 0, pixel = 0, sample = 0, ctr = 0;
+    uint32_t js = 0, pixel = 0, ctr = 0;
+    PathKey key{0u, 0u};
     int depth = 0;
     V3 shift{0, 0, 0}, next_dir{0, 0, 0}, T{0, 0, 0};
     if (i < n) {
@@ -372,8 +374,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
           ps.rad[3 * i + 2] = rad.z;
         }
         pixel = ps.pixel[i];
-        sample = ps.sample[i];
-        const PathKey key = path_key(job.seed, job.key, pixel, sample);
+        key = PathKey{ps.key_a[i], ps.key_b[i]};
         ctr = ps.ctr[i];
         // ShadeIndirect (muon/integration.cc:294-328)
         V3 dir, nT;
@@ -403,8 +404,9 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
       qn[2 * ni + 1] = make_float4(next_dir.x, next_dir.y, next_dir.z, 0.0f);
       pn.job[ni] = js;
       pn.pixel[ni] = pixel;
-      pn.sample[ni] = sample;
       pn.ctr[ni] = ctr;
+      pn.key_a[ni] = key.a;
+      pn.key_b[ni] = key.b;
       pn.depth[ni] = depth + 1;
       pn.thr[3 * ni] = T.x;
       pn.thr[3 * ni + 1] = T.y;
@@ -731,13 +733,15 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
     const uint32_t sample = ip.sample, pixel = ip.pixel;
     const int px = int(pixel % uint32_t(job.cam.width)), py = int(pixel / uint32_t(job.cam.width));
     float x, y;
-    sample_position(px, py, int(sample), path_key(job.seed, job.key, pixel, sample), x, y);
+    const PathKey key = path_key(job.seed, job.key, pixel, sample);
+    sample_position(px, py, int(sample), key, x, y);
     V3 o, d;
     cast_ray(job.cam, x, y, o, d);
     const uint32_t ni = qbase + j;
+    pn.key_a[ni] = key.a;
+    pn.key_b[ni] = key.b;
     pn.job[ni] = js;
     pn.pixel[ni] = pixel;
-    pn.sample[ni] = sample;
     pn.ctr[ni] = 0;
     pn.depth[ni] = 0;
     pn.thr[3 * ni] = 1.0f;
@@ -815,11 +819,12 @@ size_t CarvePaths(uintptr_t base, size_t P, EngArgs &a) {
   for (auto &st : a.st) {
     c(st.job);
     c(st.pixel);
-    c(st.sample);
     c(st.ctr);
     c(st.depth);
     c(st.rad);
     c(st.thr);
+    c(st.key_a);
+    c(st.key_b);
   }
   c(a.q_ray[0]);
   c(a.q_ray[1]);
@@ -1121,8 +1126,12 @@ int Engine::FlushAdmitted() {
     MX_HIP(hipMemcpyAsync(job_first_.as<unsigned long long>() + s0, h_first_ + s0,
                           sizeof(unsigned long long) * run, hipMemcpyHostToDevice, stream_));
   }
-  hipLaunchKernelGGL(k_admit_clear, dim3(kClearBlocksPerJob, uint32_t(hi - lo)), dim3(256), 0, stream_,
-                     jobs_.as<JobDev>(), uint32_t(lo % max_jobs_), max_jobs_);
+  constexpr uint64_t kMaxGridY = 65535;  // the grid's y extent holds one job each
+  for (uint64_t q = lo; q < hi; q += kMaxGridY) {
+    const uint32_t m = uint32_t(std::min<uint64_t>(hi - q, kMaxGridY));
+    hipLaunchKernelGGL(k_admit_clear, dim3(kClearBlocksPerJob, m), dim3(256), 0, stream_, jobs_.as<JobDev>(),
+                       uint32_t(q % max_jobs_), max_jobs_);
+  }
   MX_HIP(hipGetLastError());
   flushed_hi_ = hi;
   return MUONX_OK;

@@ -63,14 +63,14 @@ constexpr size_t CarveSize(size_t count, size_t elem_bytes) {
   return (count * elem_bytes + kCarveAlign - 1) & ~(kCarveAlign - 1);
 }
 
-// Path buffer: two path states (job, pixel, sample, RNG counter, depth,
-// radiance x3, throughput x3), the two ray queues and the MIS rays (two
-// float4 per path each), the hit primitive, the next-queue index, the MIS
-// factor x3, the MIS hit, the shadow / MIS origin and the hit normal (one
-// float4 each).
+// Path buffer: two path states (job, pixel, RNG counter, depth, radiance x3,
+// throughput x3, the two words of the sample's random-stream key), the two
+// ray queues and the MIS rays (two float4 per path each), the hit primitive,
+// the next-queue index, the MIS factor x3, the MIS hit, the shadow / MIS
+// origin and the hit normal (one float4 each).
 constexpr ArraySpec kPathArrays[] = {
-    {4, 1}, {4, 1}, {4, 1}, {4, 1}, {4, 1}, {4, 3}, {4, 3},  // state 0
-    {4, 1}, {4, 1}, {4, 1}, {4, 1}, {4, 1}, {4, 3}, {4, 3},  // state 1
+    {4, 1}, {4, 1}, {4, 1}, {4, 1}, {4, 3}, {4, 3}, {4, 1}, {4, 1},  // state 0
+    {4, 1}, {4, 1}, {4, 1}, {4, 1}, {4, 3}, {4, 3}, {4, 1}, {4, 1},  // state 1
     {16, 2}, {16, 2}, {16, 2},                                // ray queues 0, 1; MIS rays
     {4, 1}, {4, 1}, {4, 3}, {4, 1},                           // hit prim, next index, MIS factor, MIS hit
     {16, 1}, {16, 1}};                                        // shadow origin + material, hit normal

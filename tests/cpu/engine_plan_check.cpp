@@ -28,7 +28,7 @@ using namespace muonx;
 // at least the path budget's per-path bytes times the rows; the largest
 // shadow-slot index the engine admits fits 32 bits.
 static void CheckLayout() {
-  CHECK(PathSlotBytes(0) == 240 && PathSlotBytes(4) == 240 + 4 * 34);
+  CHECK(PathSlotBytes(0) == 248 && PathSlotBytes(4) == 248 + 4 * 34);
   for (size_t P : {size_t(1), size_t(7), size_t(1024), size_t(22800000), size_t(1) << 26}) {
     for (size_t K : {size_t(1), size_t(2), size_t(4), size_t(9)}) {
       for (int which = 0; which < 2; ++which) {
