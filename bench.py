@@ -56,8 +56,12 @@ with the library's default configuration):
   resident scenes (icospheres of 20k..1.3M tris, four of them at 1.3M, and
   a sphere field: about 5.7M primitives) at a fixed rate chosen as a load
   fraction of the batched throughput of the same request mix on those
-  scenes; images/s, p50/p99 latency (submit to film written) and the relMSE
-  of the four smallest requests against converged renders.
+  scenes (192 requests by default), replayed under each slice order
+  (oldest first, least work first, highest response ratio); images/s, p50/p99
+  latency (submit to film written), the relMSE of the four smallest
+  requests against converged renders, and each request rendered again
+  alone: its slowdown (served latency / alone latency, p50 and p99) and
+  whether every served film equals the alone one bit for bit.
 """
 import argparse
 import json
@@ -365,25 +369,19 @@ def relmse(x, r):
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
 
-def bench_serving(ctx, pool, nreq, capacity_samples_s, load, seed, log_fn):
-    import numpy as np
-    import torch
-    from muon_amd.serving import Server, engine_config, render_batch
-    from muon_amd.workloads import poisson_gaps, served_requests
-    reqs = served_requests(pool, nreq, seed)
-    mean_samples = float(np.mean([p.width * p.height * p.spp for _, p in reqs]))
-    rate = load * capacity_samples_s / mean_samples
-    gaps = poisson_gaps(nreq, rate, seed)
-    cfg = engine_config(max_jobs=256, film_budget_bytes=1 << 30, iterations_per_poll=2)
-    log_fn(f"bench: serving {nreq} requests at {rate:.2f}/s (load {load})")
+def serve_stream(ctx, reqs, gaps, order):
+    """Replays the request stream (request i submitted gaps[0..i] after the
+    start, key i) on a new server with the given slice order. Returns the
+    results, the films and the rays traced."""
+    from muon_amd.serving import Server, engine_config
+    cfg = engine_config(max_jobs=256, film_budget_bytes=1 << 30, iterations_per_poll=2, slice_order=order)
     outs = {}
     with Server(ctx, cfg) as srv:
         # warm the engine (graph capture, scene table) outside the measurement
         ds0, p0 = reqs[0]
         srv.wait(srv.submit(ds0, p0, 0), timeout_ms=-1)
         warm = srv.stats()
-        t_start = time.perf_counter()
-        t_next = t_start
+        t_next = time.perf_counter()
         tickets = []
         for i, (ds, p) in enumerate(reqs):
             t_next += gaps[i]
@@ -397,20 +395,69 @@ def bench_serving(ctx, pool, nreq, capacity_samples_s, load, seed, log_fn):
             results.append(r)
             outs[i] = acc
         stats = srv.stats()
-    t_end = max(r.done_ns for r in results)
-    t_first = min(r.submit_ns for r in results)
-    makespan = (t_end - t_first) * 1e-9
-    lat = np.array([r.latency_s for r in results])
+    rays = sum(stats[k] - warm[k] for k in ("primary_rays", "secondary_rays", "shadow_rays"))
+    return results, outs, rays
+
+
+def bench_serving(ctx, pool, nreq, capacity_samples_s, load, seed, log_fn):
+    """The served workload under each slice order (the library's default at
+    the top level, the others under other_orders), each request then
+    rendered alone for its slowdown and a bit-for-bit comparison with every
+    served film."""
+    import numpy as np
+    import torch
+    from muon_amd.serving import (SLICES_LEAST_WORK_FIRST, SLICES_OLDEST_FIRST, SLICES_RESPONSE_RATIO, Server,
+                                  engine_config)
+    from muon_amd.workloads import poisson_gaps, served_requests
+    reqs = served_requests(pool, nreq, seed)
+    mean_samples = float(np.mean([p.width * p.height * p.spp for _, p in reqs]))
+    rate = load * capacity_samples_s / mean_samples
+    gaps = poisson_gaps(nreq, rate, seed)
+    log_fn(f"bench: serving {nreq} requests at {rate:.2f}/s (load {load})")
+    runs = {}
+    orders = (("oldest_first", SLICES_OLDEST_FIRST), ("least_work_first", SLICES_LEAST_WORK_FIRST),
+              ("response_ratio", SLICES_RESPONSE_RATIO))
+    default = next(name for name, order in orders if order == engine_config().slice_order)
+    for name, order in orders:
+        runs[name] = serve_stream(ctx, reqs, gaps, order)
+        log_fn(f"bench: serving stream replayed, {name}")
+    # each request again, alone on a server with the same key: its latency
+    # without contention, and its film, which must equal both served ones
+    # bit for bit
+    alone, same = [], {name: 0 for name in runs}
+    with Server(ctx, engine_config(max_jobs=256, film_budget_bytes=1 << 30, iterations_per_poll=2)) as srv:
+        srv.wait(srv.submit(reqs[0][0], reqs[0][1], 0), timeout_ms=-1)
+        for i, (ds, p) in enumerate(reqs):
+            acc, r = srv.wait(srv.submit(ds, p, i), timeout_ms=-1)
+            alone.append(r.latency_s)
+            for name, (_, outs, _) in runs.items():
+                same[name] += int(torch.equal(acc, outs[i]))
+            del acc
+    alone = np.array(alone)
+
+    def summary(name):
+        results, _, rays = runs[name]
+        makespan = (max(r.done_ns for r in results) - min(r.submit_ns for r in results)) * 1e-9
+        lat = np.array([r.latency_s for r in results])
+        slow = lat / np.maximum(alone, 1e-9)
+        return {"images_per_s": round(nreq / makespan, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+                "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "mean_latency_s": round(float(lat.mean()), 3),
+                "mean_queue_s": round(float(np.mean([r.queue_s for r in results])), 3),
+                "slowdown_p50": round(float(np.percentile(slow, 50)), 2),
+                "slowdown_p99": round(float(np.percentile(slow, 99)), 2),
+                "identical_to_alone": f"{same[name]}/{nreq}", "mrays_per_s": round(rays / makespan / 1e6, 1)}
+
     # relMSE of the four smallest requests against converged renders
+    outs = runs[default][1]
     small = sorted(range(nreq), key=lambda i: reqs[i][1].width * reqs[i][1].height * reqs[i][1].spp)[:4]
     rel = relmse_vs_converged(ctx, [(reqs[i][0], reqs[i][1], outs[i]) for i in small], 9000)
-    return {"requests": nreq, "rate_per_s": round(rate, 3), "load": load,
-            "images_per_s": round(nreq / makespan, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
-            "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "mean_latency_s": round(float(lat.mean()), 3),
-            "mean_queue_s": round(float(np.mean([r.queue_s for r in results])), 3),
-            "mrays_per_s": round(sum(stats[k] - warm[k] for k in ("primary_rays", "secondary_rays", "shadow_rays"))
-                                 / makespan / 1e6, 1),
-            "relmse_vs_converged": rel}
+    out = {"requests": nreq, "rate_per_s": round(rate, 3), "load": load, "slice_order": default}
+    out.update(summary(default))
+    out.update({"alone_latency_p50_s": round(float(np.percentile(alone, 50)), 3),
+                "alone_latency_max_s": round(float(alone.max()), 3),
+                "other_orders": {name: summary(name) for name, _ in orders if name != default},
+                "relmse_vs_converged": rel})
+    return out
 
 
 def main():
@@ -428,7 +475,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving workloads")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=192)
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB device copy first (reference kernel for PMC byte counters)")

@@ -342,12 +342,16 @@ typedef struct muonx_engine_config {
   uint64_t film_budget_bytes;   /* HBM for in-flight films, 24 B per pixel; 0: 1 GiB */
   int32_t use_graph;            /* replay each iteration as a captured hipGraph */
   int32_t slice_order;          /* which running render gets the next slice of samples:
-                                   MUONX_SLICES_OLDEST_FIRST (0) or
+                                   MUONX_SLICES_OLDEST_FIRST (0),
                                    MUONX_SLICES_LEAST_WORK_FIRST (1: lower median latency
-                                   under load, longer tail) */
+                                   under load, longer tail) or
+                                   MUONX_SLICES_RESPONSE_RATIO (2: the highest ratio of
+                                   work handed out since the render's submission to its
+                                   own work left) */
 } muonx_engine_config;
 #define MUONX_SLICES_OLDEST_FIRST 0
 #define MUONX_SLICES_LEAST_WORK_FIRST 1
+#define MUONX_SLICES_RESPONSE_RATIO 2
 
 typedef struct muonx_job {
   muonx_scene *scene;            /* resident scene (its context = the engine's) */

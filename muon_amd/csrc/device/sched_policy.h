@@ -68,6 +68,14 @@ class RangeAllocator {
 struct SliceCandidate {
   uint64_t work_left;  // pixels x samples not yet handed to the engine (0: nothing left)
   uint32_t passes;     // slices given to newer work since its last slice, while it had work left
+  uint64_t age = 0;    // pixels x samples handed to the engine (for any render) since it was submitted
+};
+
+// Which running render gets the next slice (after forced and new renders).
+enum class SliceOrder {
+  kOldestFirst,     // arrival order
+  kLeastWorkFirst,  // least work left
+  kResponseRatio,   // highest (age + work left) / work left, oldest on ties
 };
 
 // A running render passed over this many times by newer work (a newly
@@ -82,9 +90,15 @@ enum class SliceChoice { kAdmitPending, kRunning, kNone };
 // Chooses what the next slice of the engine's backlog is for: the oldest
 // running render passed over kSlicePassLimit times (`*index`); else a
 // pending render whose film fits (its first slice); else a running render
-// with work left, the oldest (least_work_first false) or the one with the
-// least work left.
-inline SliceChoice PickSlice(const std::vector<SliceCandidate> &running, bool pending_fits, bool least_work_first,
+// with work left, by `order`. The response ratio (highest response ratio
+// next, Brinch Hansen 1971, with work in place of time: age is the work the
+// engine was given since the render's submission, which grows at the
+// engine's rate while it waits) sets a render's wait against its own
+// remaining work: a small render goes ahead of a large one soon after it
+// arrives, and a large one goes ahead once it has waited in proportion to
+// its size, so neither the median (as with oldest-first) nor the tail (as
+// with least-work-first) is given up for the other.
+inline SliceChoice PickSlice(const std::vector<SliceCandidate> &running, bool pending_fits, SliceOrder order,
                              size_t *index) {
   for (size_t k = 0; k < running.size(); ++k) {
     if (running[k].work_left && running[k].passes >= kSlicePassLimit) {
@@ -95,12 +109,23 @@ inline SliceChoice PickSlice(const std::vector<SliceCandidate> &running, bool pe
   if (pending_fits) return SliceChoice::kAdmitPending;
   bool found = false;
   for (size_t k = 0; k < running.size(); ++k) {
-    if (!running[k].work_left) continue;
-    if (!least_work_first) {
+    const SliceCandidate &c = running[k];
+    if (!c.work_left) continue;
+    if (order == SliceOrder::kOldestFirst) {
       *index = k;
       return SliceChoice::kRunning;
     }
-    if (!found || running[k].work_left < running[*index].work_left) {
+    bool better = !found;
+    if (found) {
+      const SliceCandidate &b = running[*index];
+      if (order == SliceOrder::kLeastWorkFirst) {
+        better = c.work_left < b.work_left;
+      } else {  // c.age / c.work_left > b.age / b.work_left, exactly
+        better = static_cast<unsigned __int128>(c.age) * b.work_left >
+                 static_cast<unsigned __int128>(b.age) * c.work_left;
+      }
+    }
+    if (better) {
       *index = k;
       found = true;
     }

@@ -47,6 +47,7 @@ struct SchedJob {
   uint32_t next_sample = 0;  // samples handed to the engine so far
   uint32_t open_slices = 0;  // of those, slices not finished yet
   uint32_t passes = 0;  // slices given to newer work since its last slice (PickSlice)
+  uint64_t handed_at_submit = 0;  // the scheduler's handed_ when it was submitted (PickSlice's age)
   int state = 0;  // kPending, kActive, kResolving, kDone
   muonx_job_result result{};
   hipEvent_t ev = nullptr;  // resolve finished
@@ -116,6 +117,7 @@ class Scheduler {
   std::map<uint64_t, SchedJob> jobs_;  // admitted, by ticket (arrival order)
   std::map<uint64_t, Slice> slices_;   // open slices by engine sequence
   uint64_t open_items_ = 0;            // unfinished samples of the open slices
+  uint64_t handed_ = 0;                // samples (pixels x spp) handed to the engine, all renders
   std::vector<hipEvent_t> free_events_;
   int snap_ = 0;
   bool snap_pending_[2] = {false, false};

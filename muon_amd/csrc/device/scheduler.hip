@@ -34,7 +34,8 @@ uint64_t FilmBytes(const muonx_render_params &p) {
 int Scheduler::Create(muonx_ctx *ctx, const muonx_tuning &tuning, const muonx_engine_config &in, Scheduler **out) {
   *out = nullptr;
   muonx_engine_config cfg = in;
-  if (cfg.slice_order != MUONX_SLICES_OLDEST_FIRST && cfg.slice_order != MUONX_SLICES_LEAST_WORK_FIRST) {
+  if (cfg.slice_order != MUONX_SLICES_OLDEST_FIRST && cfg.slice_order != MUONX_SLICES_LEAST_WORK_FIRST &&
+      cfg.slice_order != MUONX_SLICES_RESPONSE_RATIO) {
     SetError("engine config: unknown slice_order");
     return MUONX_ERR_INVALID;
   }
@@ -129,6 +130,7 @@ int Scheduler::Submit(const muonx_job &job, uint64_t ticket, uint64_t submit_ns)
   j.state = kPending;
   j.result.status = MUONX_PENDING;
   j.result.submit_ns = submit_ns;
+  j.handed_at_submit = handed_;
   pending_.push_back(j);
   return MUONX_OK;
 }
@@ -155,6 +157,7 @@ int Scheduler::AdmitSlice(SchedJob &j) {
   if (slices_.empty()) progress_iter_ = iterations_;
   slices_.emplace(seq, sl);
   open_items_ += sl.nitems;
+  handed_ += sl.nitems;
   j.next_sample += n;
   ++j.open_slices;
   return MUONX_OK;
@@ -172,7 +175,7 @@ int Scheduler::AdmitWork() {
       if (j.state != kActive) continue;
       const uint32_t spp = uint32_t(j.spec.params.spp);
       const uint64_t left = j.next_sample < spp ? uint64_t(spp - j.next_sample) * j.npix : 0;
-      cands.push_back(SliceCandidate{left, j.passes});
+      cands.push_back(SliceCandidate{left, j.passes, handed_ - j.handed_at_submit});
       running.push_back(&j);
     }
     // the oldest waiting job's film, if it fits (admission by film memory,
@@ -180,7 +183,10 @@ int Scheduler::AdmitWork() {
     uint64_t off = 0;
     const bool fits = !pending_.empty() && films_.Alloc(pending_.front().film_bytes, &off);
     size_t idx = 0;
-    const SliceChoice ch = PickSlice(cands, fits, cfg_.slice_order == MUONX_SLICES_LEAST_WORK_FIRST, &idx);
+    const SliceOrder order = cfg_.slice_order == MUONX_SLICES_LEAST_WORK_FIRST ? SliceOrder::kLeastWorkFirst
+                             : cfg_.slice_order == MUONX_SLICES_RESPONSE_RATIO ? SliceOrder::kResponseRatio
+                                                                               : SliceOrder::kOldestFirst;
+    const SliceChoice ch = PickSlice(cands, fits, order, &idx);
     if (ch != SliceChoice::kAdmitPending && fits) films_.Free(off, pending_.front().film_bytes);
     RecordPick(cands, ch, idx);
     for (size_t k = 0; k < running.size(); ++k) running[k]->passes = cands[k].passes;

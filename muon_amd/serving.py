@@ -14,6 +14,7 @@ from ._lib import check, lib
 
 SLICES_OLDEST_FIRST = 0      # a running render's next slice: oldest render first (default)
 SLICES_LEAST_WORK_FIRST = 1  # ... or the render with the least work left (lower median, longer tail)
+SLICES_RESPONSE_RATIO = 2    # ... or the highest ratio of work handed out since its submission to its work left
 
 
 def engine_config(path_slots=0, max_jobs=0, shadow_per_vertex=0, iterations_per_poll=0, film_budget_bytes=0,

@@ -171,17 +171,48 @@ static void CheckRanges(std::mt19937_64 &rng) {
   }
 }
 
-// Slice choice: new admissions first, then the oldest or least-work render;
-// and under a stream that always has a new render waiting, a running large
-// render still gets a slice at least every kSlicePassLimit + 1 picks
-// (without the pass count it would get none).
-static void CheckSlicePolicy() {
+// Slice choice: new admissions first, then the oldest, least-work or
+// highest-response-ratio render; and under a stream that always has a new
+// render waiting, a running large render still gets a slice at least every
+// kSlicePassLimit + 1 picks (without the pass count it would get none).
+static void CheckSlicePolicy(std::mt19937_64 &rng) {
+  constexpr SliceOrder kOldest = SliceOrder::kOldestFirst, kLeast = SliceOrder::kLeastWorkFirst,
+                       kRatio = SliceOrder::kResponseRatio;
   size_t idx = 99;
-  std::vector<SliceCandidate> c = {{50, 0}, {0, 0}, {20, 0}};
-  CHECK(PickSlice(c, true, false, &idx) == SliceChoice::kAdmitPending);
-  CHECK(PickSlice(c, false, false, &idx) == SliceChoice::kRunning && idx == 0);
-  CHECK(PickSlice(c, false, true, &idx) == SliceChoice::kRunning && idx == 2);
-  CHECK(PickSlice({{0, 0}}, false, true, &idx) == SliceChoice::kNone);
+  std::vector<SliceCandidate> c = {{50, 0, 100}, {0, 0, 500}, {20, 0, 10}};
+  CHECK(PickSlice(c, true, kOldest, &idx) == SliceChoice::kAdmitPending);
+  CHECK(PickSlice(c, true, kRatio, &idx) == SliceChoice::kAdmitPending);
+  CHECK(PickSlice(c, false, kOldest, &idx) == SliceChoice::kRunning && idx == 0);
+  CHECK(PickSlice(c, false, kLeast, &idx) == SliceChoice::kRunning && idx == 2);
+  // ratios 100/50 = 2 and 10/20 = 0.5 (render 1 has no work left)
+  CHECK(PickSlice(c, false, kRatio, &idx) == SliceChoice::kRunning && idx == 0);
+  c[2].age = 41;  // 41/20 > 2
+  CHECK(PickSlice(c, false, kRatio, &idx) == SliceChoice::kRunning && idx == 2);
+  c[2].age = 40;  // a tie: the older render
+  CHECK(PickSlice(c, false, kRatio, &idx) == SliceChoice::kRunning && idx == 0);
+  c[2].age = 10;
+  CHECK(PickSlice({{0, 0}}, false, kLeast, &idx) == SliceChoice::kNone);
+  CHECK(PickSlice({{0, 0, 7}}, false, kRatio, &idx) == SliceChoice::kNone);
+  // the ratio pick against exact rational comparison on random candidates
+  // (ages and work up to 2^40, as a large scene's samples reach)
+  for (int rep = 0; rep < 20000; ++rep) {
+    std::vector<SliceCandidate> r(1 + rng() % 9);
+    for (auto &x : r) {
+      x.work_left = rng() % 4 == 0 ? 0 : 1 + rng() % (1ull << (1 + rng() % 40));
+      x.age = rng() % (1ull << (rng() % 41));
+      x.passes = 0;
+    }
+    size_t k = 99;
+    const SliceChoice ch = PickSlice(r, false, kRatio, &k);
+    size_t best = r.size();
+    for (size_t i = 0; i < r.size(); ++i) {
+      if (!r[i].work_left) continue;
+      if (best == r.size() || static_cast<unsigned __int128>(r[i].age) * r[best].work_left >
+                                  static_cast<unsigned __int128>(r[best].age) * r[i].work_left)
+        best = i;
+    }
+    CHECK(best == r.size() ? ch == SliceChoice::kNone : (ch == SliceChoice::kRunning && k == best));
+  }
   // passes: a new admission passes every running render with work left
   RecordPick(c, SliceChoice::kAdmitPending, 0);
   CHECK(c[0].passes == 1 && c[1].passes == 0 && c[2].passes == 1);
@@ -189,17 +220,20 @@ static void CheckSlicePolicy() {
   RecordPick(c, SliceChoice::kRunning, 2);
   CHECK(c[0].passes == 2 && c[2].passes == 0);
   c[0].passes = kSlicePassLimit;
-  CHECK(PickSlice(c, true, true, &idx) == SliceChoice::kRunning && idx == 0);
-  for (int lwf = 0; lwf < 2; ++lwf) {
+  CHECK(PickSlice(c, true, kLeast, &idx) == SliceChoice::kRunning && idx == 0);
+  CHECK(PickSlice(c, true, kRatio, &idx) == SliceChoice::kRunning && idx == 0);
+  for (SliceOrder order : {kOldest, kLeast, kRatio}) {
     // one large render with 1000 slices of work; at every pick a new
-    // two-slice render is waiting and fits
-    std::vector<SliceCandidate> run = {{1000, 0}};
+    // two-slice render is waiting and fits; every pick hands out one slice,
+    // which ages every render submitted before it
+    std::vector<SliceCandidate> run = {{1000, 0, 0}};
     uint64_t last = 0, max_gap = 0, picks_large = 0;
     for (uint64_t pick = 1; pick <= 5000; ++pick) {
-      const SliceChoice ch = PickSlice(run, true, lwf != 0, &idx);
+      const SliceChoice ch = PickSlice(run, true, order, &idx);
       RecordPick(run, ch, idx);
+      for (auto &x : run) ++x.age;
       if (ch == SliceChoice::kAdmitPending) {
-        run.push_back({2, 0});
+        run.push_back({2, 0, 0});
       } else {
         CHECK(ch == SliceChoice::kRunning);
         run[idx].work_left -= 1;
@@ -239,7 +273,7 @@ int main() {
   CheckLayout();
   CheckWindow(rng);
   CheckRanges(rng);
-  CheckSlicePolicy();
+  CheckSlicePolicy(rng);
   CheckSlotRuns(rng);
   std::printf("ok\n");
   return 0;

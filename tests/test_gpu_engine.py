@@ -115,10 +115,10 @@ def test_keys_change_the_noise_not_the_estimate(ctx, resident):
     assert abs(d.mean()) < 6.0 * d.std() / np.sqrt(d.size)
 
 
-@pytest.mark.parametrize("order", [0, 1], ids=["oldest-first", "least-work-first"])
+@pytest.mark.parametrize("order", [0, 1, 2], ids=["oldest-first", "least-work-first", "response-ratio"])
 def test_server_results_equal_alone_under_load(ctx, resident, order):
     """Requests arrive while others run; the film budget admits only a few
-    at a time, so later requests wait for films to free up. Either order of
+    at a time, so later requests wait for films to free up. Every order of
     continuing slices gives every request its image alone."""
     jobs = job_list(resident) * 3
     ref = [alone(ds, p) for ds, p, _ in jobs]
@@ -324,12 +324,13 @@ def test_autotune_installs_the_fastest_configuration(resident):
     assert np.isfinite(img).all() and img.sum() > 0
 
 
-@pytest.mark.parametrize("order", [0, 1], ids=["oldest-first", "least-work-first"])
+@pytest.mark.parametrize("order", [0, 1, 2], ids=["oldest-first", "least-work-first", "response-ratio"])
 def test_server_large_job_completes_under_a_request_stream(ctx, resident, order):
     """One large render (128 slices of a 4096-path engine) while a stream
     keeps up to 8 small requests waiting: new requests' first slices go
-    first and least-work-first prefers the small ones, but a render passed
-    over 16 times by newer work goes ahead (tests/cpu/engine_plan_check.cpp
+    first and least-work-first (and, until it has waited long, the
+    response ratio) prefers the small ones, but a render passed over 16
+    times by newer work goes ahead (tests/cpu/engine_plan_check.cpp
     checks that bound), so the large render finishes while the stream still
     runs, with its image alone."""
     hs, ds = resident["cornell"]
