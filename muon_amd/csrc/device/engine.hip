@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
         const float4 qo = q[2 * i], qd = q[2 * i + 1];
         const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
         const HitRecord h = hit_record(s, uint32_t(prim), ro, rd);
-        const int32_t mi = s.prim_mat[prim];
+        const int32_t mi = ldg(s.prim_mat + prim);
         const V3 shift = h.pos + kEps * h.normal;
         // the hit for shade_bsdf: shadow/MIS origin + material, normal
         a.shift[i] = make_float4(shift.x, shift.y, shift.z, __int_as_float(mi));
@@ -125,18 +125,18 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
             int k = 0;
             for (uint32_t l = 0; l < s.num_lights; ++l) {
               const float *L = s.lights + size_t(l) * kLightStride;
-              const int kind = int(L[0]);
+              const int kind = int(ldg(L));
               if (kind != MUONX_LIGHT_QUAD) {
-                V3 ldir, col{L[1], L[2], L[3]};
+                V3 ldir, col{ldg(L + 1), ldg(L + 2), ldg(L + 3)};
                 float ldist;
                 if (kind == MUONX_LIGHT_POINT) {
-                  const V3 to_light = V3{L[4], L[5], L[6]} - h.pos;
+                  const V3 to_light = V3{ldg(L + 4), ldg(L + 5), ldg(L + 6)} - h.pos;
                   ldir = est_normalize(to_light);
                   ldist = est_length(to_light);
-                  const float att = L[7] + L[8] * ldist + L[9] * ldist * ldist;
+                  const float att = ldg(L + 7) + ldg(L + 8) * ldist + ldg(L + 9) * ldist * ldist;
                   col = est_div(col, att);
                 } else {
-                  ldir = V3{L[4], L[5], L[6]};
+                  ldir = V3{ldg(L + 4), ldg(L + 5), ldg(L + 6)};
                   ldist = __builtin_inff();
                 }
                 V3 irr = col;
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
               for (uint32_t k = 0; k < s.num_light_prims; ++k) {
                 float dist;
                 uint32_t flags;
-                if (prim_hit(s, s.light_prims[k], mr, oc, dist, flags) && dist > 0.0f && dist < dmin) dmin = dist;
+                if (prim_hit(s, ldg(s.light_prims + k), mr, oc, dist, flags) && dist > 0.0f && dist < dmin) dmin = dist;
               }
             }
             if (dmin == __builtin_inff()) {
@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
         const float4 qo = q[2 * i], qd = q[2 * i + 1];
         const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
         const HitRecord h = hit_record(s, uint32_t(prim), ro, rd);
-        const int32_t mi = s.prim_mat[prim];
+        const int32_t mi = ldg(s.prim_mat + prim);
         shift = h.pos + kEps * h.normal;
         const Mat m = load_mat(s.materials + size_t(mi) * kMatStride);
         T = V3{ps.thr[3 * i], ps.thr[3 * i + 1], ps.thr[3 * i + 2]};
@@ -642,10 +642,10 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
         const int32_t mh = a.m_hit[i];
         if (mh >= 0) {
           const DevSceneView &s = a.scenes[a.jobs[js].scene];
-          const int32_t light = s.prim_light[mh];
+          const int32_t light = ldg(s.prim_light + mh);
           if (light >= 0) {
             const float *L = s.lights + size_t(light) * kLightStride;
-            madd = V3{a.m_fac[3 * i] * L[1], a.m_fac[3 * i + 1] * L[2], a.m_fac[3 * i + 2] * L[3]};
+            madd = V3{a.m_fac[3 * i] * ldg(L + 1), a.m_fac[3 * i + 1] * ldg(L + 2), a.m_fac[3 * i + 2] * ldg(L + 3)};
           }
         }
         direct = direct + madd;

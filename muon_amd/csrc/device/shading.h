@@ -21,6 +21,7 @@
 
 #include "../common/vecmath.h"
 #include "dev_scene.h"
+#include "gload.h"
 #include "rng.h"
 
 namespace muonx {
@@ -77,13 +78,13 @@ struct Mat {
 
 __device__ __forceinline__ Mat load_mat(const float *m) {
   Mat r;
-  r.diffuse = V3{m[3], m[4], m[5]};
-  r.specular = V3{m[6], m[7], m[8]};
-  r.emission = V3{m[9], m[10], m[11]};
-  r.shininess = m[12];
-  r.roughness = m[13];
-  r.brdf = int(m[14]);
-  r.refl = m[15];
+  r.diffuse = V3{ldg(m + 3), ldg(m + 4), ldg(m + 5)};
+  r.specular = V3{ldg(m + 6), ldg(m + 7), ldg(m + 8)};
+  r.emission = V3{ldg(m + 9), ldg(m + 10), ldg(m + 11)};
+  r.shininess = ldg(m + 12);
+  r.roughness = ldg(m + 13);
+  r.brdf = int(ldg(m + 14));
+  r.refl = ldg(m + 15);
   return r;
 }
 
@@ -164,11 +165,11 @@ struct QuadLight {
 };
 __device__ __forceinline__ QuadLight load_quad(const float *L) {
   QuadLight q;
-  q.color = V3{L[1], L[2], L[3]};
-  q.corner = V3{L[4], L[5], L[6]};
-  q.e0 = V3{L[7], L[8], L[9]};
-  q.e1 = V3{L[10], L[11], L[12]};
-  q.area = L[15];
+  q.color = V3{ldg(L + 1), ldg(L + 2), ldg(L + 3)};
+  q.corner = V3{ldg(L + 4), ldg(L + 5), ldg(L + 6)};
+  q.e0 = V3{ldg(L + 7), ldg(L + 8), ldg(L + 9)};
+  q.e1 = V3{ldg(L + 10), ldg(L + 11), ldg(L + 12)};
+  q.area = ldg(L + 15);
   return q;
 }
 
@@ -197,7 +198,7 @@ __device__ __forceinline__ float nee_pdf(const DevSceneView &s, V3 o, V3 d) {
   int affecting = 0;
   for (uint32_t l = 0; l < s.num_lights; ++l) {
     const float *L = s.lights + size_t(l) * kLightStride;
-    if (int(L[0]) != MUONX_LIGHT_QUAD) continue;
+    if (int(ldg(L)) != MUONX_LIGHT_QUAD) continue;
     ++affecting;
     const QuadLight q = load_quad(L);
     const float dist = quad_hit(q, o, d);

@@ -21,6 +21,7 @@
 
 #include "../common/vecmath.h"
 #include "dev_scene.h"
+#include "gload.h"
 
 namespace muonx {
 
@@ -137,8 +138,10 @@ __device__ __forceinline__ bool sphere_test(V3 c, float rad, V3 o, V3 d, V3 &p) 
 
 // Ray.Transform(M^-1) of muon/ray.cc:7-11.
 __device__ __forceinline__ void to_object(const float *xf, V3 o, V3 d, V3 &oo, V3 &od) {
-  oo = xform_point(xf + 16, o);
-  od = normalize(xform_vec(xf + 16, d));
+  float mi[16];
+  load_xform(xf + 16, mi);
+  oo = xform_point(mi, o);
+  od = normalize(xform_vec(mi, d));
 }
 
 struct TraceCounters {
@@ -160,8 +163,8 @@ struct ObjCache {
 
 __device__ __forceinline__ bool prim_hit(const DevSceneView &s, uint32_t i, const TraceRay &r, ObjCache &oc,
                                          float &dist, uint32_t &flags) {
-  const float4 q0 = s.prims[3 * i];
-  const float4 q1 = s.prims[3 * i + 1];
+  const float4 q0 = ldg(s.prims + 3 * i);
+  const float4 q1 = ldg(s.prims + 3 * i + 1);
   const uint32_t tw = __float_as_uint(q0.w);
   flags = __float_as_uint(q1.w);
   const int32_t tid = int32_t(tw & 0x7fffffffu);
@@ -175,11 +178,13 @@ __device__ __forceinline__ bool prim_hit(const DevSceneView &s, uint32_t i, cons
   if (tw & kSphereBit) {
     hit = sphere_test(V3{q0.x, q0.y, q0.z}, q1.x, oc.o, oc.d, p);
   } else {
-    const float4 q2 = s.prims[3 * i + 2];
+    const float4 q2 = ldg(s.prims + 3 * i + 2);
     hit = tri_test(V3{q0.x, q0.y, q0.z}, V3{q1.x, q1.y, q1.z}, V3{q2.x, q2.y, q2.z}, oc.o, oc.d, p);
   }
   if (!hit) return false;
-  V3 pos = xform_point(xf, p);
+  float m[16];
+  load_xform(xf, m);
+  V3 pos = xform_point(m, p);
   dist = length(pos - r.o);
   return true;
 }
@@ -291,8 +296,8 @@ template <bool COUNT, bool EXACT, class Stack>
 __device__ __forceinline__ bool node_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
                                           uint32_t &ref, TraceCounters &cnt) {
   const float4 *np = s.nodes + 4 * size_t(ref);
-  const float4 n0 = np[0], n1 = np[1], n2 = np[2];
-  const int4 n3 = *reinterpret_cast<const int4 *>(np + 3);
+  const float4 n0 = ldg(np), n1 = ldg(np + 1), n2 = ldg(np + 2);
+  const int4 n3 = ldg(reinterpret_cast<const int4 *>(np + 3));
   if (COUNT) cnt.nodes++;
   float lo0, lo1;
   bool h0, h1;
@@ -431,14 +436,14 @@ __device__ __forceinline__ int32_t trace_bvh_s(const DevSceneView &s, const Trac
       TraceCounters none;
       return trace_bvh_s<ANY, false, EXACT, WW>(s, r, best, stack, none);
     }
-    ref = __float_as_uint(s.nodes[3].x);
+    ref = __float_as_uint(ldg(s.nodes + 3).x);
   } else if (kRefStats) {
     // The reference tests the root's box first (muon/acceleration.cc:92-99);
     // here it is the union of node 0's child boxes. A ray that misses it
     // misses every box below, so stopping changes no result. A root that is
     // a leaf sits in node 0's first child (the second is a sentinel).
     const float4 *np = s.nodes;
-    const float4 n0 = np[0], n1 = np[1], n2 = np[2];
+    const float4 n0 = ldg(np), n1 = ldg(np + 1), n2 = ldg(np + 2);
     float lo;
     cnt.btests++;
     if (!box_hit_exact(r, fminf(n0.x, n1.x), fmaxf(n0.y, n1.y), fminf(n0.z, n1.z), fmaxf(n0.w, n1.w),
@@ -492,9 +497,10 @@ template <bool COUNT, class Stack>
 __device__ __forceinline__ bool wnode_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
                                            uint32_t &ref, TraceCounters &cnt) {
   const float4 *np = s.wnodes + 8 * size_t(ref);
-  const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-  uint4 rf = *reinterpret_cast<const uint4 *>(np + 6);
-  const uint32_t n = __float_as_uint(np[7].x);
+  const float4 lx = ldg(np), hx = ldg(np + 1), ly = ldg(np + 2), hy = ldg(np + 3), lz = ldg(np + 4),
+               hz = ldg(np + 5);
+  uint4 rf = ldg(reinterpret_cast<const uint4 *>(np + 6));
+  const uint32_t n = __float_as_uint(ldg(np + 7).x);
   if (COUNT) cnt.nodes++;
   constexpr float kInf = __builtin_inff();
   float t0, t1, t2, t3;
@@ -567,8 +573,8 @@ template <bool COUNT, bool ANY, class Stack>
 __device__ __forceinline__ bool qnode_step(const DevSceneView &s, const TraceRay &r, float best, Stack &stack,
                                            uint32_t &ref, TraceCounters &cnt) {
   const uint4 *np = s.qnodes + 4 * size_t(ref);
-  const uint4 n0 = np[0], n1 = np[1], n2 = np[2];
-  uint4 rf = np[3];
+  const uint4 n0 = ldg(np), n1 = ldg(np + 1), n2 = ldg(np + 2);
+  uint4 rf = ldg(np + 3);
   if (COUNT) cnt.nodes++;
   const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
   const float sx = __uint_as_float((n0.w & 255u) << 23), sy = __uint_as_float(((n0.w >> 8) & 255u) << 23);
