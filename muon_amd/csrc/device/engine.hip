@@ -659,9 +659,9 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
       } else {
         unsigned long long *px = a.jobs[js].film + 3 * size_t(ps.pixel[i]);
         const unsigned long long fx = to_fixed(rad.x), fy = to_fixed(rad.y), fz = to_fixed(rad.z);
-        if (fx) atomicAdd(px, fx);
-        if (fy) atomicAdd(px + 1, fy);
-        if (fz) atomicAdd(px + 2, fz);
+        if (fx) atomic_add_global(px, fx);
+        if (fy) atomic_add_global(px + 1, fy);
+        if (fz) atomic_add_global(px + 2, fz);
         fin = true;
       }
     }
@@ -679,14 +679,14 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
         const uint32_t h = jl & (kDoneTable - 1);
         const uint32_t prev = atomicCAS(&tjob[h], 0xffffffffu, jl);
         if (prev == 0xffffffffu || prev == jl) atomicAdd(&tcnt[h], (unsigned long long)__popcll(same));
-        else atomicAdd(a.jobs[jl].done, (unsigned long long)__popcll(same));  // another job holds the entry
+        else atomic_add_global(a.jobs[jl].done, (unsigned long long)__popcll(same));  // another job holds the entry
       }
       pending &= ~same;
     }
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < kDoneTable; k += blockDim.x)
-    if (tcnt[k]) atomicAdd(a.jobs[tjob[k]].done, tcnt[k]);
+    if (tcnt[k]) atomic_add_global(a.jobs[tjob[k]].done, tcnt[k]);
 }
 
 // ---------------------------------------------------------------- plan / fill

@@ -47,6 +47,13 @@ __device__ __forceinline__ void load_xform(const float *xf, float m[16]) {
   }
 }
 
+// atomicAdd (relaxed, device scope) through a global-address-space pointer,
+// for counters and film sums reached through pointers read from memory.
+__device__ __forceinline__ void atomic_add_global(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_fetch_add((__attribute__((address_space(1))) unsigned long long *)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace muonx
 
 #endif
