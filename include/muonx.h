@@ -335,7 +335,7 @@ int muonx_render_host(muonx_scene *scene, const muonx_render_params *params, flo
 
 typedef struct muonx_engine_config {
   uint32_t path_slots;          /* 0: as many as tuning.path_budget_mb holds */
-  uint32_t max_jobs;            /* in-flight job slots; 0: 1024 */
+  uint32_t max_jobs;            /* in-flight job slots, at most 65536; 0: 1024 */
   int32_t shadow_per_vertex;    /* largest NEE shadow-ray count per vertex a job
                                    may need; 0: 4 */
   uint32_t iterations_per_poll; /* engine iterations between completion checks; 0: 2 */

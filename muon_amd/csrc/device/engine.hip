@@ -46,14 +46,14 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
     const uint32_t i = base + threadIdx.x;
     if (i < n) {
       const float4 qo = q[2 * i], qd = q[2 * i + 1];
-      const uint32_t js = ps.job[i];
+      const uint32_t jw = ps.job[i], js = PathJobSlot(jw);
       const int depth = ps.depth[i];
       int32_t hit = -1;
       // Integrator::Trace returns 0 past the depth cutoff (muon/integration.cc:29-33)
       if (depth <= a.jobs[js].cutoff && depth < kMaxBounces) {
         // a register copy: the traversal's global spill stores must not force
         // re-reads of the scene's pointers
-        const DevSceneView sv = a.scenes[a.jobs[js].scene];
+        const DevSceneView sv = a.scenes[PathSceneSlot(jw)];
         const TraceRay r = make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z});
         float best = __builtin_inff();
         auto st = make_stack<KIND>(lds, kEngBlock, sa);
@@ -88,8 +88,8 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
     uint64_t vmask = 0;  // valid shadow rays k < 64
     int K = 0;
     if (i < n) {
-      const uint32_t js = ps.job[i];
-      const JobDev job = a.jobs[js];
+      const uint32_t jw = ps.job[i];
+      const JobDev job = a.jobs[PathJobSlot(jw)];
       const int32_t prim = a.hit_prim[i];
       const int depth = ps.depth[i];
       K = job.K;
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
       }
       a.m_hit[i] = -1;
       if (prim >= 0) {
-        const DevSceneView s = a.scenes[job.scene];
+        const DevSceneView s = a.scenes[PathSceneSlot(jw)];
         const float4 qo = q[2 * i], qd = q[2 * i + 1];
         const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
         const HitRecord h = hit_record(s, uint32_t(prim), ro, rd);
@@ -228,9 +228,9 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
     float mis_tmax = __builtin_inff();
     V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0}, T{0, 0, 0};
     if (i < n && a.hit_prim[i] >= 0) {
-      js = ps.job[i];
-      const JobDev job = a.jobs[js];
-      const DevSceneView s = a.scenes[job.scene];
+      js = ps.job[i];  // the job word: a continuing path carries it on
+      const JobDev job = a.jobs[PathJobSlot(js)];
+      const DevSceneView s = a.scenes[PathSceneSlot(js)];
       depth = ps.depth[i];
       const float4 sh = a.shift[i], hn = a.hnorm[i];
       shift = V3{sh.x, sh.y, sh.z};
@@ -355,9 +355,9 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
       a.m_hit[i] = -1;  // retire: no MIS term
       const int32_t prim = a.hit_prim[i];
       if (prim >= 0) {
-        js = ps.job[i];
-        const JobDev job = a.jobs[js];
-        const DevSceneView s = a.scenes[job.scene];
+        js = ps.job[i];  // the job word: a continuing path carries it on
+        const JobDev job = a.jobs[PathJobSlot(js)];
+        const DevSceneView s = a.scenes[PathSceneSlot(js)];
         depth = ps.depth[i];
         const float4 qo = q[2 * i], qd = q[2 * i + 1];
         const V3 ro{qo.x, qo.y, qo.z}, rd{qd.x, qd.y, qd.z};
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_shadow(EngArgs a, StackArgs s
     if (j < n) {
       const uint32_t sl = a.s_queue[j];
       const uint32_t i = sl % P;
-      const DevSceneView sv = a.scenes[a.jobs[ps.job[i]].scene];
+      const DevSceneView sv = a.scenes[PathSceneSlot(ps.job[i])];
       const float4 so = a.shift[i], sd = a.s_dir[sl];
       const TraceRay r = make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z});
       float best = sd.w;
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_mis(EngArgs a, StackArgs sa, 
     if (j < n) {
       const float4 mo = a.m_ray[2 * j], md = a.m_ray[2 * j + 1];
       const uint32_t i = __float_as_uint(mo.w);
-      const DevSceneView s = a.scenes[a.jobs[ps.job[i]].scene];
+      const DevSceneView s = a.scenes[PathSceneSlot(ps.job[i])];
       const TraceRay r = make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z});
       float best = md.w;  // the nearest emitter hit, or +inf
       auto st = make_stack<KIND>(lds, kEngBlock, sa);
@@ -493,11 +493,11 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
       if (need) {
         if (k < n) {
           i = k;
-          const uint32_t js = ps.job[i];
+          const uint32_t jw = ps.job[i], js = PathJobSlot(jw);
           const int depth = ps.depth[i];
           // Integrator::Trace returns 0 past the depth cutoff (muon/integration.cc:29-33)
           if (depth <= a.jobs[js].cutoff && depth < kMaxBounces) {
-            sv = a.scenes[a.jobs[js].scene];
+            sv = a.scenes[PathSceneSlot(jw)];
             const float4 qo = q[2 * i], qd = q[2 * i + 1];
             lt.start(make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z}), __builtin_inff(), st);
             have = true;
@@ -544,7 +544,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
         if (k < n) {
           sl = a.s_queue[k];
           const uint32_t i = sl % P;
-          sv = a.scenes[a.jobs[ps.job[i]].scene];
+          sv = a.scenes[PathSceneSlot(ps.job[i])];
           const float4 so = a.shift[i], sd = a.s_dir[sl];
           lt.start(make_ray(V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}), sd.w, st);
           have = true;
@@ -585,7 +585,7 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
         if (k < n) {
           const float4 mo = a.m_ray[2 * k], md = a.m_ray[2 * k + 1];
           i = __float_as_uint(mo.w);
-          sv = a.scenes[a.jobs[ps.job[i]].scene];
+          sv = a.scenes[PathSceneSlot(ps.job[i])];
           lt.start(make_ray(V3{mo.x, mo.y, mo.z}, V3{md.x, md.y, md.z}), md.w, st);
           have = true;
           ++traced;
@@ -627,7 +627,8 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
     bool fin = false;
     uint32_t js = 0xffffffffu;
     if (i < n) {
-      js = ps.job[i];
+      const uint32_t jw = ps.job[i];
+      js = PathJobSlot(jw);
       V3 rad{ps.rad[3 * i], ps.rad[3 * i + 1], ps.rad[3 * i + 2]};
       if (a.hit_prim[i] >= 0) {
         const int K = a.jobs[js].K;
@@ -641,7 +642,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
         V3 madd{0.0f, 0.0f, 0.0f};
         const int32_t mh = a.m_hit[i];
         if (mh >= 0) {
-          const DevSceneView &s = a.scenes[a.jobs[js].scene];
+          const DevSceneView &s = a.scenes[PathSceneSlot(jw)];
           const int32_t light = ldg(s.prim_light + mh);
           if (light >= 0) {
             const float *L = s.lights + size_t(light) * kLightStride;
@@ -740,7 +741,7 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_fill(EngArgs a, int cur) {
     const uint32_t ni = qbase + j;
     pn.key_a[ni] = key.a;
     pn.key_b[ni] = key.b;
-    pn.job[ni] = js;
+    pn.job[ni] = PackPathJob(js, job.scene);
     pn.pixel[ni] = pixel;
     pn.ctr[ni] = 0;
     pn.depth[ni] = 0;
@@ -900,6 +901,10 @@ int Engine::Create(muonx_ctx *ctx, const muonx_tuning &tuning, uint32_t capacity
     SetError("engine: capacity, max_jobs and kmax must be positive");
     return MUONX_ERR_INVALID;
   }
+  if (max_jobs > kMaxPathSlotIndex) {  // a path's job word holds the job slot in 16 bits
+    SetError("engine: max_jobs must be at most 65536");
+    return MUONX_ERR_INVALID;
+  }
   if (capacity > (1u << 30) || uint64_t(capacity) * uint64_t(kmax) > kMaxShadowSlots) {
     SetError("engine: paths in flight x shadow rays per vertex must stay below 2^31");
     return MUONX_ERR_INVALID;
@@ -1002,6 +1007,10 @@ int Engine::AddScene(muonx_scene *scene, uint32_t *slot) {
     }
   }
   if (at == scene_ids_.size()) {
+    if (at >= kMaxPathSlotIndex) {  // a path's job word holds the scene slot in 16 bits
+      SetError("engine: 65536 live scenes in use");
+      return MUONX_ERR_CAPACITY;
+    }
     scene_ids_.push_back(scene->id);
     host_scenes_.push_back(scene->view);
     scene_ptrs_.push_back(scene);

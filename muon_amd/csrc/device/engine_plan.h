@@ -21,6 +21,15 @@ constexpr uint32_t FillCount(unsigned long long total_items, unsigned long long 
   return uint32_t(avail < room ? avail : room);
 }
 
+// A path's job word: its job slot in the low 16 bits and the job's scene
+// slot in the high 16, so that the kernels fetch the job record and the
+// scene view in parallel instead of reading the scene slot from the job
+// record first. The engine keeps max_jobs and the scene slots below 2^16.
+constexpr uint32_t kMaxPathSlotIndex = 1u << 16;
+constexpr uint32_t PackPathJob(uint32_t job_slot, uint32_t scene_slot) { return job_slot | (scene_slot << 16); }
+constexpr uint32_t PathJobSlot(uint32_t word) { return word & 0xffffu; }
+constexpr uint32_t PathSceneSlot(uint32_t word) { return word >> 16; }
+
 // Sequence number of the admitted job whose work holds global item `item`:
 // the last of the jobs [lo, hi) whose first item is <= item. First items
 // grow with the sequence number; job_first is a ring indexed by

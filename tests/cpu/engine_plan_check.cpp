@@ -248,6 +248,17 @@ static void CheckSlicePolicy(std::mt19937_64 &rng) {
   }
 }
 
+// A path's job word: job and scene slot below 2^16 each come back out.
+static void CheckPathJobWord(std::mt19937_64 &rng) {
+  for (int rep = 0; rep < 100000; ++rep) {
+    const uint32_t job = uint32_t(rng() % kMaxPathSlotIndex), scene = uint32_t(rng() % kMaxPathSlotIndex);
+    const uint32_t w = PackPathJob(job, scene);
+    CHECK(PathJobSlot(w) == job && PathSceneSlot(w) == scene);
+  }
+  CHECK(PathJobSlot(PackPathJob(kMaxPathSlotIndex - 1, kMaxPathSlotIndex - 1)) == kMaxPathSlotIndex - 1);
+  CHECK(PathSceneSlot(PackPathJob(0, kMaxPathSlotIndex - 1)) == kMaxPathSlotIndex - 1);
+}
+
 // The admission flush's slot runs: every job of [lo, hi) exactly once, in
 // job order, inside the ring, in at most two runs.
 static void CheckSlotRuns(std::mt19937_64 &rng) {
@@ -274,6 +285,7 @@ int main() {
   CheckWindow(rng);
   CheckRanges(rng);
   CheckSlicePolicy(rng);
+  CheckPathJobWord(rng);
   CheckSlotRuns(rng);
   std::printf("ok\n");
   return 0;

@@ -184,6 +184,13 @@ def test_server_cancel(ctx, resident):
     assert st["cancelled"] == 1 and st["completed"] == 2 and st["film_bytes_in_use"] == 0
 
 
+def test_server_rejects_more_than_65536_job_slots(ctx):
+    """A path's job word holds its job slot in 16 bits."""
+    with pytest.raises(Exception, match="max_jobs"):
+        with Server(ctx, engine_config(path_slots=4096, max_jobs=65537)):
+            pass
+
+
 def test_server_rejects_unknown_slice_order(ctx):
     with pytest.raises(Exception, match="slice_order"):
         with Server(ctx, engine_config(path_slots=4096, slice_order=7)):
