@@ -36,7 +36,7 @@
 namespace muonx {
 
 constexpr int kEngBlock = 128;
-constexpr int kShadeBlock = 256;  // shade kernels: one queue-append atomic per block
+constexpr int kShadeBlock = 512;  // shade kernels: one queue-append atomic per block and append
 constexpr int kShadeWaves = kShadeBlock / 64;
 constexpr int kMaxBounces = 1024;  // hard cap for max_depth = -1 without Russian roulette
 constexpr double kFilmScale = 16777216.0;  // fixed-point units per radiance unit

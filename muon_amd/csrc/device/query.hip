@@ -64,7 +64,11 @@ __global__ void __launch_bounds__(kQueryBlock) __attribute__((amdgpu_waves_per_e
   auto st = make_stack<KIND>(lds, kQueryBlock, sa);
   LaneTrace<WIDTH, ANY, COUNT> lt;
   uint32_t idx = 0;
-  WaveFetch wf(n);
+  // chunks of at most 128 rays: 2^24 closest-hit rays over the level-7
+  // icosphere trace in 3.245 ms against 3.285 ms with 256 (and 3.41 ms with
+  // 512, 3.45 ms with 64); the engine's kernels keep 256, with which the
+  // path-traced render of the same scene is 4 % faster than with 128
+  WaveFetch wf(n, 128);
   bool have = false, drained = false;
   while (true) {
     const bool need = !have && !drained;

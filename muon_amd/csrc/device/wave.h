@@ -96,8 +96,9 @@ __device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) 
 
 // Work claiming for kernels that give each lane a new item as soon as its
 // current one is finished. Items are handed out in chunks of `chunk`
-// consecutive indices (64..kMaxChunk, about four chunks per wavefront of the
-// grid): wavefront w's first chunk is [w * chunk, (w + 1) * chunk) with no
+// consecutive indices (64..max_chunk, about four chunks per wavefront of the
+// grid; larger chunks leave more work in a few wavefronts' last chunks,
+// smaller ones put more atomics on the one counter): wavefront w's first chunk is [w * chunk, (w + 1) * chunk) with no
 // atomic, later chunks come from a global counter (one atomic per chunk,
 // offset past the first chunks of all wavefronts). A wavefront whose first
 // chunk lies past the item count exits without touching the counter, so a
@@ -106,11 +107,11 @@ __device__ __forceinline__ void wave_count(uint32_t n, unsigned long long *ctr) 
 struct WaveFetch {
   static constexpr uint32_t kMaxChunk = 256;
   uint32_t chunk, first, cbase, cend;
-  __device__ __forceinline__ explicit WaveFetch(uint32_t items) {
+  __device__ __forceinline__ explicit WaveFetch(uint32_t items, uint32_t max_chunk = kMaxChunk) {
     const uint32_t wpb = (blockDim.x + 63u) >> 6;
     const uint32_t waves = gridDim.x * wpb;
     const uint32_t c = (items / (4u * waves)) & ~63u;
-    chunk = c < 64u ? 64u : (c > kMaxChunk ? kMaxChunk : c);
+    chunk = c < 64u ? 64u : (c > max_chunk ? max_chunk : c);
     first = waves * chunk;
     cbase = (blockIdx.x * wpb + (threadIdx.x >> 6)) * chunk;
     cend = cbase + chunk;
