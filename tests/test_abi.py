@@ -82,3 +82,20 @@ def test_python_struct_mirrors_match_the_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(cls), cname
         for field, _ in cls._fields_:
             assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
+
+
+def test_python_constants_mirror_the_header():
+    """The integer constants the Python package restates carry the header's
+    values: _lib's MUONX_* names, serving's SLICES_* orders (MUONX_SLICES_*),
+    and the traversal mode DeviceScene.query passes as int(exact)."""
+    import re
+    from muon_amd import _lib, serving
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "include", "muonx.h")).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define (MUONX_\w+) \(?(-?\d+)\)?", text)}
+    mirrored = [(n, getattr(_lib, n)) for n in dir(_lib) if n.startswith("MUONX_")]
+    mirrored += [("MUONX_" + n, getattr(serving, n)) for n in dir(serving) if n.startswith("SLICES_")]
+    assert len(mirrored) >= 5
+    for name, value in mirrored:
+        assert defs[name] == value, name
+    assert defs["MUONX_TRAVERSAL_FAST"] == int(False) and defs["MUONX_TRAVERSAL_EXACT"] == int(True)
