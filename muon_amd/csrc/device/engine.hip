@@ -70,6 +70,24 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_extend(EngArgs a, StackArgs s
 }
 
 // ---------------------------------------------------------------- shade
+// The job word and hit primitive of a thread's next path in a grid-stride
+// loop, loaded one pass ahead: the chain of dependent fetches of the next
+// path (job record and scene view, then primitive, transform, material)
+// starts from registers instead of waiting for these two loads first.
+struct PathAhead {
+  uint32_t jw = 0;
+  int32_t prim = -1;
+  __device__ __forceinline__ PathAhead(const PathState &ps, const int32_t *hit_prim, uint32_t i, uint32_t n) {
+    next(ps, hit_prim, i, n);
+  }
+  __device__ __forceinline__ void next(const PathState &ps, const int32_t *hit_prim, uint32_t i, uint32_t n) {
+    if (i < n) {
+      jw = ps.job[i];
+      prim = hit_prim[i];
+    }
+  }
+};
+
 // PathTracer::Shade (muon/integration.cc:205-271) in two kernels, split where
 // the random draws of the light samples end, so each holds fewer registers:
 //   shade_light  hit record, emission, NEE shadow rays (ShadeDirectNEE)
@@ -82,15 +100,18 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
   const float4 *q = a.q_ray[cur];
   const PathState &ps = a.st[cur];
   const size_t P = a.capacity;
-  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+  const uint32_t stride = gridDim.x * kShadeBlock;
+  PathAhead ahead(ps, a.hit_prim, blockIdx.x * kShadeBlock + threadIdx.x, n);
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
+    const uint32_t jw = ahead.jw;
+    const int32_t prim = ahead.prim;
+    ahead.next(ps, a.hit_prim, i + stride, n);
     uint32_t nshadow = 0;
     uint64_t vmask = 0;  // valid shadow rays k < 64
     int K = 0;
     if (i < n) {
-      const uint32_t jw = ps.job[i];
       const JobDev job = a.jobs[PathJobSlot(jw)];
-      const int32_t prim = a.hit_prim[i];
       const int depth = ps.depth[i];
       K = job.K;
       for (int k = 0; k < K; ++k) {
@@ -219,16 +240,21 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
   float4 *qn = a.q_ray[cur ^ 1];
   const PathState &ps = a.st[cur];
   const PathState &pn = a.st[cur ^ 1];
-  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+  const uint32_t stride = gridDim.x * kShadeBlock;
+  PathAhead ahead(ps, a.hit_prim, blockIdx.x * kShadeBlock + threadIdx.x, n);
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
+    const uint32_t jw = ahead.jw;
+    const int32_t prim = ahead.prim;
+    ahead.next(ps, a.hit_prim, i + stride, n);
     bool cont = false, want_mis = false, culled = false;
     uint32_t js = 0, pixel = 0, ctr = 0;
     PathKey key{0u, 0u};
     int depth = 0;
     float mis_tmax = __builtin_inff();
     V3 shift{0, 0, 0}, next_dir{0, 0, 0}, mis_dir{0, 0, 0}, T{0, 0, 0};
-    if (i < n && a.hit_prim[i] >= 0) {
-      js = ps.job[i];  // the job word: a continuing path carries it on
+    if (i < n && prim >= 0) {
+      js = jw;  // the job word: a continuing path carries it on
       const JobDev job = a.jobs[PathJobSlot(js)];
       const DevSceneView s = a.scenes[PathSceneSlot(js)];
       depth = ps.depth[i];
@@ -344,8 +370,13 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
   float4 *qn = a.q_ray[cur ^ 1];
   const PathState &ps = a.st[cur];
   const PathState &pn = a.st[cur ^ 1];
-  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+  const uint32_t stride = gridDim.x * kShadeBlock;
+  PathAhead ahead(ps, a.hit_prim, blockIdx.x * kShadeBlock + threadIdx.x, n);
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
+    const uint32_t jw = ahead.jw;
+    const int32_t prim = ahead.prim;
+    ahead.next(ps, a.hit_prim, i + stride, n);
     bool cont = false;
     uint32_t js = 0, pixel = 0, ctr = 0;
     PathKey key{0u, 0u};
@@ -353,9 +384,8 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
     V3 shift{0, 0, 0}, next_dir{0, 0, 0}, T{0, 0, 0};
     if (i < n) {
       a.m_hit[i] = -1;  // retire: no MIS term
-      const int32_t prim = a.hit_prim[i];
       if (prim >= 0) {
-        js = ps.job[i];  // the job word: a continuing path carries it on
+        js = jw;  // the job word: a continuing path carries it on
         const JobDev job = a.jobs[PathJobSlot(js)];
         const DevSceneView s = a.scenes[PathSceneSlot(js)];
         depth = ps.depth[i];
