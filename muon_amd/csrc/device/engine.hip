@@ -525,11 +525,15 @@ __global__ void __launch_bounds__(kEngBlock) __attribute__((amdgpu_waves_per_eu(
           i = k;
           const uint32_t jw = ps.job[i], js = PathJobSlot(jw);
           const int depth = ps.depth[i];
+          // the ray and scene view are fetched with the job's cutoff, not
+          // after it: every queued path has both, and starting the lane's
+          // walk only sets registers, so it happens whether or not the
+          // cutoff lets the ray be traced
+          const float4 qo = q[2 * i], qd = q[2 * i + 1];
+          sv = a.scenes[PathSceneSlot(jw)];
+          lt.start(make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z}), __builtin_inff(), st);
           // Integrator::Trace returns 0 past the depth cutoff (muon/integration.cc:29-33)
           if (depth <= a.jobs[js].cutoff && depth < kMaxBounces) {
-            sv = a.scenes[PathSceneSlot(jw)];
-            const float4 qo = q[2 * i], qd = q[2 * i + 1];
-            lt.start(make_ray(V3{qo.x, qo.y, qo.z}, V3{qd.x, qd.y, qd.z}), __builtin_inff(), st);
             have = true;
             if (depth == 0) ++prim_rays;
             else ++sec_rays;
