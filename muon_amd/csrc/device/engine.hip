@@ -728,20 +728,30 @@ __global__ void __launch_bounds__(kEngBlock) k_eng_retire(EngArgs a, int cur) {
 // After retire: the queue just consumed is empty; new samples go behind the
 // continuing paths of the next queue.
 __global__ void k_eng_plan(EngArgs a, int cur) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (blockIdx.x != 0) return;
   EngCounters *c = a.c;
+  // the ray-count slots folded by the wavefront's lanes, one slot each (the
+  // slots sit on separate cache lines; a single thread walking them one
+  // after another was the kernel's whole cost)
+  static_assert(kRaySlots <= 64, "one lane per slot");
+  const int lane = int(threadIdx.x);
+  unsigned long long v[kRayCounters], tot[kRayCounters];
+#pragma unroll
+  for (int id = 0; id < kRayCounters; ++id) {  // every load first, then the sums
+    v[id] = lane < kRaySlots ? c->ray_slots[id][lane].v : 0ull;
+    tot[id] = lane == 0 ? c->rays[id] : 0ull;
+  }
+#pragma unroll
+  for (int id = 0; id < kRayCounters; ++id) {
+    if (lane < kRaySlots) c->ray_slots[id][lane].v = 0;
+    for (int off = 32; off > 0; off >>= 1) v[id] += __shfl_down(v[id], off);
+    if (lane == 0) c->rays[id] = tot[id] + v[id];
+  }
+  if (lane != 0) return;
   c->qcount[cur].v = 0;
   c->mis_count.v = 0;
   c->shadow_count.v = 0;
   c->claim[0].v = c->claim[1].v = c->claim[2].v = 0;
-  for (int id = 0; id < kRayCounters; ++id) {
-    unsigned long long sum = 0;
-    for (int k = 0; k < kRaySlots; ++k) {
-      sum += c->ray_slots[id][k].v;
-      c->ray_slots[id][k].v = 0;
-    }
-    c->rays[id] += sum;
-  }
   const uint32_t next = c->qcount[cur ^ 1].v;
   const uint32_t nfill = FillCount(c->total_items, c->cursor, a.capacity, next);
   c->plan_n = nfill;
